@@ -1,0 +1,74 @@
+"""ctypes binding of libbehavert_amd.so (the C ABI in include/behavert_amd.h).
+
+The library is built in-tree by behavert_amd/build.py.  Loading fails loudly when
+it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from .types import GridParams, ProxParams, SteerParams, StepBuffers
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbehavert_amd.so")
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_F = C.c_float
+_I = C.c_int
+_GP = C.POINTER(GridParams)
+_PP = C.POINTER(ProxParams)
+_SP = C.POINTER(SteerParams)
+
+# name -> (restype, argtypes); every symbol include/behavert_amd.h declares
+SIGNATURES = {
+    "brt_context_create": (_I, [_I, C.POINTER(_P)]),
+    "brt_context_destroy": (_I, [_P]),
+    "brt_context_last_error": (C.c_char_p, [_P]),
+    "brt_version": (C.c_char_p, []),
+    "brt_grid_setup": (_I, [C.POINTER(_F), C.POINTER(_F), C.POINTER(_F), _U32, _GP]),
+    "brt_list_stride": (_U32, [_U32]),
+    "brt_radius_threshold": (_F, [_F]),
+    "brt_envgrid_calc_hash": (_I, [_P, _GP, _P, _U32, _P, _P]),
+    "brt_envgrid_sort": (_I, [_P, _P, _U32, _U32, _P]),
+    "brt_envgrid_hash_sort": (_I, [_P, _GP, _P, _U32, _P, _P]),
+    "brt_envgrid_reorder": (_I, [_P, _P, _U32, _U32, C.POINTER(_P), C.POINTER(_P), _P]),
+    "brt_prox_cell_ranges": (_I, [_P, _P, _U32, _U32, _P, _P, _P]),
+    "brt_envgrid_reorder_ranges": (_I, [_P, _P, _U32, _U32, C.POINTER(_P), C.POINTER(_P), _U32, _P, _P, _P]),
+    "brt_prox_neighbors": (_I, [_P, _GP, _PP, _P, _P, _P, _P, _U32, _P, _P]),
+    "brt_osw_steer_integrate": (_I, [_P, _GP, _PP, _SP, _F, _U32, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     _P, _P]),
+    "brt_bc_feature_update": (_I, [_P, _PP, _F, _U32, _P, _P, _P, _P, _P]),
+    "brt_step": (_I, [_P, _GP, _PP, _SP, _F, _F, _U32, C.POINTER(StepBuffers), _P]),
+    "brt_phase_timing_enable": (_I, [_P, _I]),
+    "brt_phase_times": (_I, [_P, C.POINTER(C.c_double), C.POINTER(_U32)]),
+}
+
+_lib = None
+
+
+class BrtError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"behavert_amd error {code}: {message}")
+        self.code = code
+
+
+def is_built() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib() -> C.CDLL:
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not is_built():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python behavert_amd/build.py` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib

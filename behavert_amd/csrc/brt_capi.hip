@@ -1,0 +1,512 @@
+// brt_capi.hip -- the extern "C" boundary (include/behavert_amd.h).
+//
+// Replaces BehaveRT's DeviceInterface/kernelCall launch path
+// (Core/DeviceInterface.cu:380-458): explicit pointers and parameters instead of a
+// __constant__ repository upload per call, a caller-chosen stream instead of the
+// default stream, and status codes instead of exit() on error.
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/behavert_amd.h"
+#include "brt_internal.hpp"
+
+struct brt_context {
+    int device = 0;
+    std::string err;
+    void* tmp = nullptr;        // sort ping-pong buffer (uint2[n])
+    size_t tmp_bytes = 0;
+    uint32_t* meta = nullptr;   // histograms, tile counters, look-back status
+    size_t meta_bytes = 0;
+    bool timing = false;
+    std::vector<std::array<hipEvent_t, BRT_NUM_PHASES + 1>> events;
+    size_t events_used = 0;
+    double phase_ms[BRT_NUM_PHASES] = {0, 0, 0, 0, 0};
+    uint32_t phase_steps = 0;
+};
+
+namespace {
+
+int set_err(brt_context* ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int hip_err(brt_context* ctx, hipError_t e, const char* what) {
+    if (e == hipErrorOutOfMemory) return set_err(ctx, BRT_ERR_OUT_OF_MEMORY, "%s: %s", what, hipGetErrorString(e));
+    return set_err(ctx, BRT_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define BRT_HIP(call, what)                              \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return hip_err(ctx, e_, what); \
+    } while (0)
+
+#define BRT_REQUIRE(cond, ...)                                              \
+    do {                                                                    \
+        if (!(cond)) return set_err(ctx, BRT_ERR_INVALID_ARGUMENT, __VA_ARGS__); \
+    } while (0)
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+int use_device(brt_context* ctx) {
+    int cur = -1;
+    BRT_HIP(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != ctx->device) BRT_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    return BRT_OK;
+}
+
+int check_grid(brt_context* ctx, const brt_grid_params* g) {
+    BRT_REQUIRE(g != nullptr, "grid params are NULL");
+    uint64_t cells = 1;
+    for (int a = 0; a < 3; ++a) {
+        BRT_REQUIRE(g->grid_size[a] >= 1, "grid_size[%d] must be >= 1", a);
+        BRT_REQUIRE(g->grid_size[a] < (1u << 23), "grid_size[%d] must be < 2^23 (24-bit cell arithmetic)", a);
+        BRT_REQUIRE(g->cell_size[a] > 0.0f, "cell_size[%d] must be > 0", a);
+        cells *= g->grid_size[a];
+    }
+    BRT_REQUIRE(cells == g->num_cells, "num_cells (%u) != product of grid_size (%llu)", g->num_cells,
+                (unsigned long long)cells);
+    BRT_REQUIRE(cells < (1ull << 31), "too many cells (%llu)", (unsigned long long)cells);
+    BRT_REQUIRE(g->max_bodies_per_cell >= 1, "max_bodies_per_cell must be >= 1");
+    return BRT_OK;
+}
+
+int check_n(brt_context* ctx, uint32_t n) {
+    BRT_REQUIRE(n < (1u << 30), "n = %u exceeds the 2^30 agent limit", n);
+    return BRT_OK;
+}
+
+int check_prox(brt_context* ctx, const brt_prox_params* p) {
+    BRT_REQUIRE(p != nullptr, "proximity params are NULL");
+    BRT_REQUIRE(p->max_neighbors >= 1 && p->max_neighbors <= BRT_MAX_NEIGHBORS,
+                "max_neighbors must be in 1..%d (got %u)", BRT_MAX_NEIGHBORS, p->max_neighbors);
+    return BRT_OK;
+}
+
+int check_steer(brt_context* ctx, const brt_steer_params* s) {
+    BRT_REQUIRE(s != nullptr, "steer params are NULL");
+    BRT_REQUIRE(s->num_targets <= BRT_MAX_TARGETS, "num_targets must be <= %d", BRT_MAX_TARGETS);
+    BRT_REQUIRE(s->mass != 0.0f, "mass must be non-zero");
+    return BRT_OK;
+}
+
+int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan) {
+    size_t need_tmp = (size_t)n * 8u;
+    if (need_tmp > ctx->tmp_bytes) {
+        if (ctx->tmp) BRT_HIP(hipFree(ctx->tmp), "hipFree(sort tmp)");
+        ctx->tmp = nullptr;
+        ctx->tmp_bytes = 0;
+        size_t bytes = need_tmp + need_tmp / 4 + 4096;
+        BRT_HIP(hipMalloc(&ctx->tmp, bytes), "hipMalloc(sort tmp)");
+        ctx->tmp_bytes = bytes;
+    }
+    size_t need_meta = brt::sort_meta_words(plan) * 4u;
+    if (need_meta > ctx->meta_bytes) {
+        if (ctx->meta) BRT_HIP(hipFree(ctx->meta), "hipFree(sort meta)");
+        ctx->meta = nullptr;
+        ctx->meta_bytes = 0;
+        size_t bytes = need_meta + need_meta / 4 + 4096;
+        BRT_HIP(hipMalloc((void**)&ctx->meta, bytes), "hipMalloc(sort meta)");
+        ctx->meta_bytes = bytes;
+    }
+    return BRT_OK;
+}
+
+// hash (+ histograms) then the onesweep passes; the sorted pairs end in `hash`.
+int run_hash_sort(brt_context* ctx, const brt_grid_params* g, const float* pos, uint32_t n, uint32_t* hash,
+                  uint32_t* cell_start, uint32_t* cell_end, hipStream_t s) {
+    brt::SortPlan plan = brt::make_sort_plan(n, g->num_cells);
+    int rc = ensure_scratch(ctx, n, plan);
+    if (rc) return rc;
+    BRT_HIP(brt::launch_prepare(ctx->meta, brt::sort_meta_words(plan), cell_start, cell_end, g->num_cells, s),
+            "k_prepare");
+    uint2* A = reinterpret_cast<uint2*>(hash);
+    uint2* B = reinterpret_cast<uint2*>(ctx->tmp);
+    uint2* first = (plan.passes & 1u) ? B : A;
+    BRT_HIP(brt::launch_hash_hist(*g, reinterpret_cast<const float4*>(pos), n, first, ctx->meta, plan, s),
+            "k_hash_hist");
+    uint2* src = first;
+    uint2* dst = first == A ? B : A;
+    for (uint32_t q = 0; q < plan.passes; ++q) {
+        BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, ctx->meta, s), "k_onesweep");
+        std::swap(src, dst);
+    }
+    return BRT_OK;
+}
+
+void fold_events(brt_context* ctx) {
+    for (size_t k = 0; k < ctx->events_used; ++k) {
+        auto& ev = ctx->events[k];
+        (void)hipEventSynchronize(ev[BRT_NUM_PHASES]);
+        for (int p = 0; p < BRT_NUM_PHASES; ++p) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[p], ev[p + 1]) == hipSuccess) ctx->phase_ms[p] += ms;
+        }
+        ctx->phase_steps += 1;
+    }
+    ctx->events_used = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* brt_version(void) { return "behavert_amd 0.1 (gfx950)"; }
+
+int brt_context_create(int device, brt_context** out) {
+    if (!out) return BRT_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return BRT_ERR_HIP;
+    if (device < 0 || device >= count) return BRT_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return BRT_ERR_HIP;
+    brt_context* ctx = new brt_context();
+    ctx->device = device;
+    *out = ctx;
+    return BRT_OK;
+}
+
+int brt_context_destroy(brt_context* ctx) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    for (auto& ev : ctx->events)
+        for (auto e : ev) (void)hipEventDestroy(e);
+    if (ctx->tmp) (void)hipFree(ctx->tmp);
+    if (ctx->meta) (void)hipFree(ctx->meta);
+    delete ctx;
+    return BRT_OK;
+}
+
+const char* brt_context_last_error(const brt_context* ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
+
+int brt_grid_setup(const float world_center[3], const float world_radius[3], const float cell_size[3],
+                   uint32_t max_bodies_per_cell, brt_grid_params* out) {
+    if (!world_center || !world_radius || !cell_size || !out) return BRT_ERR_INVALID_ARGUMENT;
+    std::memset(out, 0, sizeof(*out));
+    uint64_t cells = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (!(cell_size[a] > 0.0f)) return BRT_ERR_INVALID_ARGUMENT;
+        // getAugmentedRadius + setupGrid: augR = R + 2*cell; size = (uint)(2*augR/cell)
+        const float aug = world_radius[a] + cell_size[a] * 2.0f;
+        const float cellsf = aug * 2.0f / cell_size[a];
+        if (!(cellsf >= 1.0f) || cellsf >= 8388608.0f) return BRT_ERR_INVALID_ARGUMENT;
+        out->grid_size[a] = (uint32_t)cellsf;
+        out->world_origin[a] = world_center[a] - aug;
+        out->cell_size[a] = cell_size[a];
+        out->world_center[a] = world_center[a];
+        out->world_radius[a] = world_radius[a];
+        cells *= out->grid_size[a];
+    }
+    if (cells >= (1ull << 31)) return BRT_ERR_INVALID_ARGUMENT;
+    out->num_cells = (uint32_t)cells;
+    out->max_bodies_per_cell = max_bodies_per_cell ? max_bodies_per_cell : BRT_UNLIMITED;
+    return BRT_OK;
+}
+
+uint32_t brt_list_stride(uint32_t max_neighbors) { return brt::list_stride(max_neighbors); }
+
+float brt_radius_threshold(float r) {
+    if (r != r) return INFINITY;
+    if (r < 0.0f) return -INFINITY;
+    if (r == INFINITY) return INFINITY;
+    float t = r * r;
+    if (t == INFINITY) t = 3.40282347e38f;
+    while (t > 0.0f && sqrtf(t) > r) t = nextafterf(t, -INFINITY);
+    while (t < INFINITY && sqrtf(nextafterf(t, INFINITY)) <= r) t = nextafterf(t, INFINITY);
+    return t;
+}
+
+int brt_envgrid_calc_hash(brt_context* ctx, const brt_grid_params* grid, const float* pos, uint32_t n,
+                          uint32_t* hash, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_grid(ctx, grid);
+    if (rc) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(pos && hash && aligned16(pos) && ((uintptr_t)hash & 7u) == 0, "bad pos/hash pointer");
+    if ((rc = use_device(ctx))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    brt::SortPlan plan = brt::make_sort_plan(n, grid->num_cells);
+    if ((rc = ensure_scratch(ctx, n, plan))) return rc;
+    // the histogram side product lands in scratch and is ignored
+    BRT_HIP(brt::launch_prepare(ctx->meta, brt::kMetaStatus, nullptr, nullptr, 0, s), "k_prepare");
+    BRT_HIP(brt::launch_hash_hist(*grid, reinterpret_cast<const float4*>(pos), n, reinterpret_cast<uint2*>(hash),
+                                  ctx->meta, plan, s),
+            "k_hash_hist");
+    return BRT_OK;
+}
+
+int brt_envgrid_sort(brt_context* ctx, uint32_t* hash, uint32_t n, uint32_t num_cells, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_n(ctx, n);
+    if (rc) return rc;
+    BRT_REQUIRE(num_cells >= 1, "num_cells must be >= 1");
+    if (n < 2) return BRT_OK;
+    BRT_REQUIRE(hash && ((uintptr_t)hash & 7u) == 0, "bad hash pointer");
+    if ((rc = use_device(ctx))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    brt::SortPlan plan = brt::make_sort_plan(n, num_cells);
+    if (plan.passes == 0) return BRT_OK;
+    if ((rc = ensure_scratch(ctx, n, plan))) return rc;
+    BRT_HIP(brt::launch_prepare(ctx->meta, brt::sort_meta_words(plan), nullptr, nullptr, 0, s), "k_prepare");
+    uint2* A = reinterpret_cast<uint2*>(hash);
+    uint2* B = reinterpret_cast<uint2*>(ctx->tmp);
+    const bool odd = plan.passes & 1u;
+    BRT_HIP(brt::launch_hist(A, odd ? B : nullptr, n, ctx->meta, plan, s), "k_hist");
+    uint2* src = odd ? B : A;
+    uint2* dst = odd ? A : B;
+    for (uint32_t q = 0; q < plan.passes; ++q) {
+        BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, ctx->meta, s), "k_onesweep");
+        std::swap(src, dst);
+    }
+    return BRT_OK;
+}
+
+int brt_envgrid_hash_sort(brt_context* ctx, const brt_grid_params* grid, const float* pos, uint32_t n,
+                          uint32_t* hash, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_grid(ctx, grid);
+    if (rc) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(pos && hash && aligned16(pos) && ((uintptr_t)hash & 7u) == 0, "bad pos/hash pointer");
+    if ((rc = use_device(ctx))) return rc;
+    return run_hash_sort(ctx, grid, pos, n, hash, nullptr, nullptr, (hipStream_t)stream);
+}
+
+static int reorder_common(brt_context* ctx, const uint32_t* hash, uint32_t n, uint32_t nf, const float* const* in,
+                          float* const* out, uint32_t num_cells, uint32_t* cell_start, uint32_t* cell_end,
+                          void* stream) {
+    int rc = check_n(ctx, n);
+    if (rc) return rc;
+    BRT_REQUIRE(nf <= (uint32_t)brt::kMaxReorderFeatures, "at most %d features per call", brt::kMaxReorderFeatures);
+    BRT_REQUIRE(nf == 0 || (in && out), "feature pointer arrays are NULL");
+    brt::ReorderArgs f{};
+    f.count = nf;
+    for (uint32_t q = 0; q < nf; ++q) {
+        BRT_REQUIRE(in[q] && out[q] && aligned16(in[q]) && aligned16(out[q]), "feature %u pointer bad", q);
+        BRT_REQUIRE(in[q] != out[q], "feature %u: in and out must differ (gather)", q);
+        f.in[q] = reinterpret_cast<const float4*>(in[q]);
+        f.out[q] = reinterpret_cast<float4*>(out[q]);
+    }
+    if ((rc = use_device(ctx))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (cell_start) {
+        BRT_REQUIRE(cell_end && num_cells >= 1, "cell_end missing");
+        BRT_REQUIRE(aligned16(cell_start) && aligned16(cell_end), "cell arrays must be 16-byte aligned");
+        BRT_HIP(brt::launch_prepare(nullptr, 0, cell_start, cell_end, num_cells, s), "k_prepare(cells)");
+    }
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(hash != nullptr, "hash is NULL");
+    BRT_HIP(brt::launch_reorder_ranges(reinterpret_cast<const uint2*>(hash), n, f, num_cells, cell_start, cell_end, s),
+            "k_reorder_ranges");
+    return BRT_OK;
+}
+
+int brt_envgrid_reorder(brt_context* ctx, const uint32_t* hash, uint32_t n, uint32_t num_features,
+                        const float* const* in, float* const* out, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    return reorder_common(ctx, hash, n, num_features, in, out, 0, nullptr, nullptr, stream);
+}
+
+int brt_prox_cell_ranges(brt_context* ctx, const uint32_t* hash, uint32_t n, uint32_t num_cells,
+                         uint32_t* cell_start, uint32_t* cell_end, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    if (!cell_start || !cell_end) return set_err(ctx, BRT_ERR_INVALID_ARGUMENT, "cell arrays are NULL");
+    return reorder_common(ctx, hash, n, 0, nullptr, nullptr, num_cells, cell_start, cell_end, stream);
+}
+
+int brt_envgrid_reorder_ranges(brt_context* ctx, const uint32_t* hash, uint32_t n, uint32_t num_features,
+                               const float* const* in, float* const* out, uint32_t num_cells,
+                               uint32_t* cell_start, uint32_t* cell_end, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    if (!cell_start || !cell_end) return set_err(ctx, BRT_ERR_INVALID_ARGUMENT, "cell arrays are NULL");
+    return reorder_common(ctx, hash, n, num_features, in, out, num_cells, cell_start, cell_end, stream);
+}
+
+int brt_prox_neighbors(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
+                       const float* sorted_pos, const uint32_t* hash, const uint32_t* cell_start,
+                       const uint32_t* cell_end, uint32_t n, uint32_t* lists, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_grid(ctx, grid);
+    if (rc) return rc;
+    if ((rc = check_prox(ctx, prox))) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(sorted_pos && hash && cell_start && cell_end && lists, "NULL array");
+    BRT_REQUIRE(aligned16(sorted_pos) && aligned16(lists), "sorted_pos and lists must be 16-byte aligned");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_neighbors(*grid, *prox, brt_radius_threshold(prox->search_radius),
+                                  reinterpret_cast<const float4*>(sorted_pos), reinterpret_cast<const uint2*>(hash),
+                                  cell_start, cell_end, n, lists, (hipStream_t)stream),
+            "k_neighbors");
+    return BRT_OK;
+}
+
+int brt_osw_steer_integrate(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
+                            const brt_steer_params* steer, float dt, uint32_t step, uint32_t n,
+                            const float* sorted_pos, const float* sorted_forward, const float* sorted_smooth_accel,
+                            const uint32_t* hash, const uint32_t* lists, float* wander, float* pos_out,
+                            float* forward_out, float* smooth_accel_out, float* force_out, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_grid(ctx, grid);
+    if (rc) return rc;
+    if ((rc = check_prox(ctx, prox))) return rc;
+    if ((rc = check_steer(ctx, steer))) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(sorted_pos && sorted_forward && sorted_smooth_accel && hash && lists && pos_out && forward_out &&
+                    smooth_accel_out,
+                "NULL array");
+    BRT_REQUIRE(aligned16(sorted_pos) && aligned16(sorted_forward) && aligned16(sorted_smooth_accel) &&
+                    aligned16(lists) && aligned16(pos_out) && aligned16(forward_out) &&
+                    aligned16(smooth_accel_out) && aligned16(force_out),
+                "float4 arrays must be 16-byte aligned");
+    BRT_REQUIRE(steer->wander_weight == 0.0f || wander != nullptr, "wander_weight != 0 needs the wander state");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_steer(*grid, prox->max_neighbors, *steer, dt, step, n,
+                              reinterpret_cast<const float4*>(sorted_pos),
+                              reinterpret_cast<const float4*>(sorted_forward),
+                              reinterpret_cast<const float4*>(sorted_smooth_accel),
+                              reinterpret_cast<const uint2*>(hash), lists, reinterpret_cast<float2*>(wander),
+                              reinterpret_cast<float4*>(pos_out), reinterpret_cast<float4*>(forward_out),
+                              reinterpret_cast<float4*>(smooth_accel_out), reinterpret_cast<float4*>(force_out),
+                              (hipStream_t)stream),
+            "k_steer");
+    return BRT_OK;
+}
+
+int brt_bc_feature_update(brt_context* ctx, const brt_prox_params* prox, float rate, uint32_t n,
+                          const uint32_t* hash, const uint32_t* lists, const float* features_in,
+                          float* features_out, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_prox(ctx, prox);
+    if (rc) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(hash && lists && features_in && features_out, "NULL array");
+    BRT_REQUIRE(features_in != features_out, "features_in and features_out must differ");
+    BRT_REQUIRE(aligned16(features_in) && aligned16(features_out) && aligned16(lists), "alignment");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_features(prox->max_neighbors, rate, n, reinterpret_cast<const uint2*>(hash), lists,
+                                 reinterpret_cast<const float4*>(features_in),
+                                 reinterpret_cast<float4*>(features_out), (hipStream_t)stream),
+            "k_features");
+    return BRT_OK;
+}
+
+int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
+             const brt_steer_params* steer, float feature_rate, float dt, uint32_t step,
+             const brt_step_buffers* b, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_grid(ctx, grid);
+    if (rc) return rc;
+    if ((rc = check_prox(ctx, prox))) return rc;
+    if ((rc = check_steer(ctx, steer))) return rc;
+    BRT_REQUIRE(b != nullptr, "buffers are NULL");
+    if ((rc = check_n(ctx, b->n))) return rc;
+    const uint32_t n = b->n;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(b->pos && b->forward && b->smooth_accel && b->sorted_pos && b->sorted_forward &&
+                    b->sorted_smooth_accel && b->hash && b->cell_start && b->cell_end && b->lists,
+                "a required step buffer is NULL");
+    BRT_REQUIRE(aligned16(b->pos) && aligned16(b->forward) && aligned16(b->smooth_accel) &&
+                    aligned16(b->sorted_pos) && aligned16(b->sorted_forward) && aligned16(b->sorted_smooth_accel) &&
+                    aligned16(b->lists) && aligned16(b->features) && aligned16(b->features_out) &&
+                    aligned16(b->cell_start) && aligned16(b->cell_end),
+                "float4 buffers must be 16-byte aligned");
+    const bool do_features = b->features && b->features_out && feature_rate != 0.0f;
+    BRT_REQUIRE(!do_features || b->features != b->features_out, "features and features_out must differ");
+    BRT_REQUIRE(steer->wander_weight == 0.0f || b->wander != nullptr, "wander_weight != 0 needs the wander state");
+    if ((rc = use_device(ctx))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+
+    std::array<hipEvent_t, BRT_NUM_PHASES + 1>* ev = nullptr;
+    if (ctx->timing) {
+        if (ctx->events_used == 4096) fold_events(ctx);
+        if (ctx->events_used == ctx->events.size()) {
+            std::array<hipEvent_t, BRT_NUM_PHASES + 1> a;
+            for (auto& e : a) BRT_HIP(hipEventCreate(&e), "hipEventCreate");
+            ctx->events.push_back(a);
+        }
+        ev = &ctx->events[ctx->events_used++];
+        BRT_HIP(hipEventRecord((*ev)[0], s), "hipEventRecord");
+    }
+    // phase 0: prepare (scratch + empty-cell fill) + hash + sort
+    if ((rc = run_hash_sort(ctx, grid, b->pos, n, b->hash, b->cell_start, b->cell_end, s))) return rc;
+    if (ev) BRT_HIP(hipEventRecord((*ev)[1], s), "hipEventRecord");
+    // phase 1: reorder pos/forward/smooth_accel + cell ranges
+    brt::ReorderArgs f{};
+    f.count = 3;
+    f.in[0] = reinterpret_cast<const float4*>(b->pos);
+    f.out[0] = reinterpret_cast<float4*>(b->sorted_pos);
+    f.in[1] = reinterpret_cast<const float4*>(b->forward);
+    f.out[1] = reinterpret_cast<float4*>(b->sorted_forward);
+    f.in[2] = reinterpret_cast<const float4*>(b->smooth_accel);
+    f.out[2] = reinterpret_cast<float4*>(b->sorted_smooth_accel);
+    BRT_HIP(brt::launch_reorder_ranges(reinterpret_cast<const uint2*>(b->hash), n, f, grid->num_cells, b->cell_start,
+                                       b->cell_end, s),
+            "k_reorder_ranges");
+    if (ev) BRT_HIP(hipEventRecord((*ev)[2], s), "hipEventRecord");
+    // phase 2: neighbour lists
+    BRT_HIP(brt::launch_neighbors(*grid, *prox, brt_radius_threshold(prox->search_radius),
+                                  reinterpret_cast<const float4*>(b->sorted_pos),
+                                  reinterpret_cast<const uint2*>(b->hash), b->cell_start, b->cell_end, n, b->lists, s),
+            "k_neighbors");
+    if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
+    // phase 3: fused steering + integration, scattered back to agent-ID order
+    BRT_HIP(brt::launch_steer(*grid, prox->max_neighbors, *steer, dt, step, n,
+                              reinterpret_cast<const float4*>(b->sorted_pos),
+                              reinterpret_cast<const float4*>(b->sorted_forward),
+                              reinterpret_cast<const float4*>(b->sorted_smooth_accel),
+                              reinterpret_cast<const uint2*>(b->hash), b->lists,
+                              reinterpret_cast<float2*>(b->wander), reinterpret_cast<float4*>(b->pos),
+                              reinterpret_cast<float4*>(b->forward), reinterpret_cast<float4*>(b->smooth_accel),
+                              nullptr, s),
+            "k_steer");
+    if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
+    // phase 4: BehaviorClustering feature update
+    if (do_features)
+        BRT_HIP(brt::launch_features(prox->max_neighbors, feature_rate, n, reinterpret_cast<const uint2*>(b->hash),
+                                     b->lists, reinterpret_cast<const float4*>(b->features),
+                                     reinterpret_cast<float4*>(b->features_out), s),
+                "k_features");
+    if (ev) BRT_HIP(hipEventRecord((*ev)[5], s), "hipEventRecord");
+    return BRT_OK;
+}
+
+int brt_phase_timing_enable(brt_context* ctx, int enable) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    ctx->timing = enable != 0;
+    for (double& v : ctx->phase_ms) v = 0.0;
+    ctx->phase_steps = 0;
+    ctx->events_used = 0;
+    return BRT_OK;
+}
+
+int brt_phase_times(brt_context* ctx, double* ms_out, uint32_t* steps_out) {
+    if (!ctx || !ms_out) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    fold_events(ctx);
+    for (int p = 0; p < BRT_NUM_PHASES; ++p) ms_out[p] = ctx->phase_ms[p];
+    if (steps_out) *steps_out = ctx->phase_steps;
+    return BRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,314 @@
+// brt_envgrid.hip -- EnvGrid3D on gfx950: cell hash, stable key/ID radix sort,
+// reorder of the per-agent float4 features and the cell ranges.
+//
+// Reference semantics:
+//   genCalcHashD ........... PlugIns/EnvGrid3D/envgrid3d_kernel.cu:55-68
+//   RadixSort (stable) ..... PlugIns/EnvGrid3D/radixsort.cu:50-80 (4 x 8-bit LSD passes,
+//                            values = agent IDs in order, so the result is the
+//                            (key, ID) lexicographic order)
+//   reorderDataFloat4 ...... PlugIns/EnvGrid3D/envgrid3d_kernel.cu:110-122
+//   genericFindCellStartD .. PlugIns/Proximity3D/Proximity3D_kernel.cu:54-84
+//
+// Design (MI355X): the sort is a onesweep LSD radix sort.  The hash kernel also
+// builds the digit histograms of every pass; each pass is then ONE kernel: a tile
+// of 4096 pairs ranks its keys per wave64 with 64-bit ballots (a digit "match"
+// mask per lane), combines the four waves' counts in LDS, publishes its per-digit
+// count and finds its global offset by decoupled look-back over the preceding
+// tiles' status words, then scatters through LDS so each digit's run is written
+// with consecutive addresses.  Only ceil(log2(cells)/8) passes run: keys are
+// < cells, so the higher digits are all zero and cannot change a stable order.
+#include "brt_device.hpp"
+#include "brt_internal.hpp"
+
+namespace brt {
+
+namespace {
+
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagInc = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1u;
+
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_tmp) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    uint32_t wo = 0;
+    for (int w = 0; w < wave; ++w) wo += s_tmp[w];
+    __syncthreads();
+    return wo + x - v;
+}
+
+__global__ void __launch_bounds__(256) k_prepare(uint32_t* __restrict__ meta, uint32_t meta_words,
+                                                 uint32_t* __restrict__ cell_start,
+                                                 uint32_t* __restrict__ cell_end, uint32_t num_cells) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < meta_words; i += stride) meta[i] = 0u;
+    if (cell_start) {
+        // four cells per thread with 16-byte stores where aligned
+        const uint32_t n4 = num_cells / 4u;
+        uint4 e = make_uint4(BRT_EMPTY_CELL, BRT_EMPTY_CELL, BRT_EMPTY_CELL, BRT_EMPTY_CELL);
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+            reinterpret_cast<uint4*>(cell_start)[i] = e;
+            reinterpret_cast<uint4*>(cell_end)[i] = e;
+        }
+        for (uint32_t i = n4 * 4u + blockIdx.x * blockDim.x + threadIdx.x; i < num_cells; i += stride) {
+            cell_start[i] = BRT_EMPTY_CELL;
+            cell_end[i] = BRT_EMPTY_CELL;
+        }
+    }
+}
+
+// genCalcHashD fused with the per-pass digit histograms.
+__global__ void __launch_bounds__(256) k_hash_hist(brt_grid_params g, const float4* __restrict__ pos,
+                                                   uint32_t n, uint2* __restrict__ out,
+                                                   uint32_t* __restrict__ hist, SortPlan plan) {
+    __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+    for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) s_hist[t] = 0u;
+    __syncthreads();
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float4 p = pos[i];
+        uint32_t key = calcGridHash(calcGridPos(p.x, p.y, p.z, g), g);
+        out[i] = make_uint2(key, i);
+        for (uint32_t q = 0; q < plan.passes; ++q)
+            atomicAdd(&s_hist[q * kRadix + ((key >> plan.shift[q]) & ((1u << plan.nbits[q]) - 1u))], 1u);
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) {
+        uint32_t v = s_hist[t];
+        if (v) atomicAdd(&hist[t], v);
+    }
+}
+
+// Histograms of existing pairs (standalone sort); optionally copies the pairs.
+__global__ void __launch_bounds__(256) k_hist(const uint2* __restrict__ in, uint2* __restrict__ copy_out,
+                                              uint32_t n, uint32_t* __restrict__ hist, SortPlan plan) {
+    __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+    for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) s_hist[t] = 0u;
+    __syncthreads();
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint2 v = in[i];
+        if (copy_out) copy_out[i] = v;
+        for (uint32_t q = 0; q < plan.passes; ++q)
+            atomicAdd(&s_hist[q * kRadix + ((v.x >> plan.shift[q]) & ((1u << plan.nbits[q]) - 1u))], 1u);
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) {
+        uint32_t v = s_hist[t];
+        if (v) atomicAdd(&hist[t], v);
+    }
+}
+
+// One onesweep pass.  Items are held warp-striped: item k of lane l of wave w is
+// element tile_base + w*64*ITEMS + k*64 + l, so (w, k, l) order is input order and
+// ranking in that order keeps the sort stable.
+template <int ITEMS>
+__global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restrict__ in, uint2* __restrict__ out,
+                                                            uint32_t n, uint32_t shift, uint32_t nbits,
+                                                            const uint32_t* __restrict__ ghist,
+                                                            uint32_t* __restrict__ status,
+                                                            uint32_t* __restrict__ tile_counter) {
+    constexpr int TILE = kSortThreads * ITEMS;
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_whist[4][kRadix];
+    __shared__ uint32_t s_digit_start[kRadix];
+    __shared__ uint32_t s_base[kRadix];
+    __shared__ uint32_t s_tmp[4];
+    __shared__ uint2 s_items[TILE];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const uint32_t mask = (1u << nbits) - 1u;
+
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) s_whist[w][tid] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t wbase = tile * TILE + wave * (64 * ITEMS);
+
+    uint2 it[ITEMS];
+    uint32_t rank[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        uint32_t idx = wbase + k * 64 + lane;
+        it[k] = idx < n ? in[idx] : make_uint2(0u, 0u);
+    }
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const bool valid = (wbase + k * 64 + lane) < n;
+        const uint32_t d = (it[k].x >> shift) & mask;
+        uint64_t m = __ballot(valid);
+        for (uint32_t b = 0; b < nbits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            m &= bit ? bal : ~bal;
+        }
+        const uint32_t cnt = __popcll(m);
+        const uint32_t below = __popcll(m & lt);
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t prev = 0;
+        if (valid) prev = s_whist[wave][d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && lane == leader) s_whist[wave][d] = prev + cnt;
+        __builtin_amdgcn_wave_barrier();
+        rank[k] = prev + below;
+    }
+    __syncthreads();
+
+    // digit = tid: per-wave exclusive offsets and the tile's count
+    const uint32_t c0 = s_whist[0][tid], c1 = s_whist[1][tid], c2 = s_whist[2][tid], c3 = s_whist[3][tid];
+    const uint32_t tcount = c0 + c1 + c2 + c3;
+    s_whist[0][tid] = 0u;
+    s_whist[1][tid] = c0;
+    s_whist[2][tid] = c0 + c1;
+    s_whist[3][tid] = c0 + c1 + c2;
+    uint32_t* my_status = status + (size_t)tile * kRadix + tid;
+    if (tile == 0)
+        __hip_atomic_store(my_status, kFlagInc | tcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        __hip_atomic_store(my_status, kFlagAgg | tcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+    const uint32_t digit_start = block_excl_scan_256(tcount, s_tmp);
+    const uint32_t gex = block_excl_scan_256(ghist[tid], s_tmp);
+    s_digit_start[tid] = digit_start;
+
+    // decoupled look-back over the preceding tiles for digit tid
+    uint32_t excl = 0u;
+    if (tile > 0) {
+        int32_t p = (int32_t)tile - 1;
+        uint32_t spins = 0;
+        while (p >= 0) {
+            uint32_t v = __hip_atomic_load(status + (size_t)p * kRadix + tid, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t flag = v & ~kValueMask;
+            if (flag == 0u) {
+                // A predecessor tile has started (tile IDs are taken in start order)
+                // and publishes without waiting, so this wait ends; the bound only
+                // turns an impossible stall into a wrong result instead of a hang.
+                if (++spins > (1u << 24)) {
+                    __hip_atomic_store(tile_counter + 8, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += v & kValueMask;
+            if (flag == kFlagInc) break;
+            --p;
+        }
+        __hip_atomic_store(my_status, kFlagInc | (excl + tcount), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_base[tid] = gex + excl;
+    __syncthreads();
+
+    // local scatter into digit order, then coalesced write-out
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const bool valid = (wbase + k * 64 + lane) < n;
+        if (valid) {
+            const uint32_t d = (it[k].x >> shift) & mask;
+            s_items[s_digit_start[d] + s_whist[wave][d] + rank[k]] = it[k];
+        }
+    }
+    __syncthreads();
+    const uint32_t tile_begin = tile * TILE;
+    const uint32_t tile_n = (n - tile_begin) < (uint32_t)TILE ? (n - tile_begin) : (uint32_t)TILE;
+    for (uint32_t j = tid; j < tile_n; j += kSortThreads) {
+        const uint2 v = s_items[j];
+        const uint32_t d = (v.x >> shift) & mask;
+        const uint32_t g = s_base[d] + (j - s_digit_start[d]);
+        if (g < n) out[g] = v;
+    }
+}
+
+// reorderDataFloat4 for every feature + cell start/end of the sorted keys.
+__global__ void __launch_bounds__(256) k_reorder_ranges(const uint2* __restrict__ hash, uint32_t n, ReorderArgs f,
+                                                        uint32_t num_cells, uint32_t* __restrict__ cell_start,
+                                                        uint32_t* __restrict__ cell_end) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint2 h = hash[i];
+    if (h.y < n)
+        for (uint32_t q = 0; q < f.count; ++q) f.out[q][i] = f.in[q][h.y];
+    if (cell_start && h.x < num_cells) {
+        if (i == 0 || hash[i - 1].x != h.x) cell_start[h.x] = i;
+        if (i == n - 1 || hash[i + 1].x != h.x) cell_end[h.x] = i + 1;
+    }
+}
+
+inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t cap) {
+    uint32_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return g < cap ? g : cap;
+}
+
+}  // namespace
+
+SortPlan make_sort_plan(uint32_t n, uint32_t num_cells) {
+    SortPlan p{};
+    uint32_t bits = 0;
+    while (bits < 32 && ((num_cells - 1u) >> bits)) ++bits;
+    p.passes = (bits + 7u) / 8u;
+    for (uint32_t q = 0; q < p.passes; ++q) {
+        p.shift[q] = 8u * q;
+        p.nbits[q] = (bits - 8u * q) < 8u ? (bits - 8u * q) : 8u;
+    }
+    p.tiles = (n + kSortTile - 1) / kSortTile;
+    return p;
+}
+
+size_t sort_meta_words(const SortPlan& p) {
+    return (size_t)kMetaStatus + (size_t)p.passes * p.tiles * kRadix;
+}
+
+hipError_t launch_prepare(uint32_t* meta, size_t meta_words, uint32_t* cell_start, uint32_t* cell_end,
+                          uint32_t num_cells, hipStream_t s) {
+    size_t work = meta_words > (size_t)num_cells / 4 ? meta_words : (size_t)num_cells / 4;
+    uint32_t grid = grid_for((uint32_t)(work > 0xFFFFFFFFull ? 0xFFFFFFFFu : work), 256, 2048);
+    hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(256), 0, s, meta, (uint32_t)meta_words, cell_start, cell_end,
+                       num_cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash_hist(const brt_grid_params& g, const float4* pos, uint32_t n, uint2* out, uint32_t* meta,
+                            const SortPlan& plan, hipStream_t s) {
+    uint32_t grid = grid_for(n, 256, 1024);
+    hipLaunchKernelGGL(k_hash_hist, dim3(grid), dim3(256), 0, s, g, pos, n, out, meta + kMetaHist, plan);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist(const uint2* in, uint2* copy_out, uint32_t n, uint32_t* meta, const SortPlan& plan,
+                       hipStream_t s) {
+    uint32_t grid = grid_for(n, 256, 1024);
+    hipLaunchKernelGGL(k_hist, dim3(grid), dim3(256), 0, s, in, copy_out, n, meta + kMetaHist, plan);
+    return hipGetLastError();
+}
+
+hipError_t launch_onesweep(const uint2* in, uint2* out, uint32_t n, uint32_t pass, const SortPlan& plan,
+                           uint32_t* meta, hipStream_t s) {
+    uint32_t* status = meta + kMetaStatus + (size_t)pass * plan.tiles * kRadix;
+    hipLaunchKernelGGL(k_onesweep<kSortItems>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n,
+                       plan.shift[pass], plan.nbits[pass], meta + kMetaHist + pass * kRadix, status,
+                       meta + kMetaCounters + pass);
+    return hipGetLastError();
+}
+
+hipError_t launch_reorder_ranges(const uint2* hash, uint32_t n, const ReorderArgs& f, uint32_t num_cells,
+                                 uint32_t* cell_start, uint32_t* cell_end, hipStream_t s) {
+    uint32_t grid = (n + 255) / 256;
+    hipLaunchKernelGGL(k_reorder_ranges, dim3(grid), dim3(256), 0, s, hash, n, f, num_cells, cell_start, cell_end);
+    return hipGetLastError();
+}
+
+}  // namespace brt

@@ -1,0 +1,64 @@
+// brt_internal.hpp -- host-side launchers shared between the kernel TUs and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/behavert_amd.h"
+
+namespace brt {
+
+constexpr int kSortThreads = 256;            // 4 wave64s per tile
+constexpr int kSortItems = 16;               // keys per thread
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kRadix = 256;                  // 8-bit digits
+constexpr int kMaxPasses = 4;
+constexpr uint32_t kMetaHist = 0;            // uint32 hist[4][256]
+constexpr uint32_t kMetaCounters = 1024;     // uint32 tile counters[4] (padded to 64)
+constexpr uint32_t kMetaStatus = 1088;       // uint32 status[passes][tiles][256]
+
+struct SortPlan {
+    uint32_t passes;
+    uint32_t tiles;
+    uint32_t shift[kMaxPasses];
+    uint32_t nbits[kMaxPasses];
+};
+
+SortPlan make_sort_plan(uint32_t n, uint32_t num_cells);
+size_t sort_meta_words(const SortPlan& p);
+
+// envgrid
+hipError_t launch_prepare(uint32_t* meta, size_t meta_words, uint32_t* cell_start, uint32_t* cell_end,
+                          uint32_t num_cells, hipStream_t s);
+hipError_t launch_hash_hist(const brt_grid_params& g, const float4* pos, uint32_t n, uint2* out,
+                            uint32_t* meta, const SortPlan& plan, hipStream_t s);
+hipError_t launch_hist(const uint2* in, uint2* copy_out, uint32_t n, uint32_t* meta, const SortPlan& plan,
+                       hipStream_t s);
+hipError_t launch_onesweep(const uint2* in, uint2* out, uint32_t n, uint32_t pass, const SortPlan& plan,
+                           uint32_t* meta, hipStream_t s);
+
+constexpr int kMaxReorderFeatures = 8;
+struct ReorderArgs {
+    const float4* in[kMaxReorderFeatures];
+    float4* out[kMaxReorderFeatures];
+    uint32_t count;
+};
+hipError_t launch_reorder_ranges(const uint2* hash, uint32_t n, const ReorderArgs& f, uint32_t num_cells,
+                                 uint32_t* cell_start, uint32_t* cell_end, hipStream_t s);
+
+// proximity
+hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, float threshold,
+                            const float4* spos, const uint2* hash, const uint32_t* cell_start,
+                            const uint32_t* cell_end, uint32_t n, uint32_t* lists, hipStream_t s);
+
+// opensteer
+hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t step,
+                        uint32_t n, const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
+                        const uint32_t* lists, float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
+                        float4* force_out, hipStream_t s);
+hipError_t launch_features(uint32_t K, float rate, uint32_t n, const uint2* hash, const uint32_t* lists,
+                           const float4* fin, float4* fout, hipStream_t s);
+
+inline uint32_t list_stride(uint32_t K) { return 4u * (K / 4u + 1u); }
+
+}  // namespace brt

@@ -1,0 +1,226 @@
+"""Python host layer over the C ABI: a context and a flock whose device buffers are
+torch tensors (torch is used only for device memory and the current HIP stream).
+
+Mirrors the reference's per-step plug-in calls (Samples/OpensteerDemo/plugins/
+BehaveRT_Flocking.cpp:177-196):
+  Flock.reorder_sim_data()       EnvGrid3DPlugIn::reorderSimData
+  Flock.compute_neighborhoods()  Proximity3DPlugIn::computeNeighborhoods
+  Flock.apply_steering(dt)       OpenSteerWrapperPlugIn::resetSteerForce ...
+                                 applySteeringForces (fused)
+  Flock.update_features(rate)    BehaviorClustering feature update
+  Flock.step(dt, i)              the whole step in one C call (brt_step)
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import BrtError, lib
+from .types import NUM_PHASES, PHASE_NAMES, GridParams, ProxParams, SteerParams, StepBuffers, list_stride
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensors passed to the C ABI must be contiguous")
+    return t.data_ptr()
+
+
+def current_stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+class Context:
+    """Owns a brt_context (sort scratch, phase timers) on one device."""
+
+    def __init__(self, device: int = 0):
+        self._lib = lib()
+        h = C.c_void_p()
+        rc = self._lib.brt_context_create(device, C.byref(h))
+        if rc != 0:
+            raise BrtError(rc, f"brt_context_create(device={device}) failed")
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            self._lib.brt_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int):
+        if rc != 0:
+            msg = self._lib.brt_context_last_error(self.handle)
+            raise BrtError(rc, msg.decode() if msg else "")
+
+    # ------------------------------------------------------------ EnvGrid3D
+    def calc_hash(self, grid: GridParams, pos, hash_out):
+        self.check(self._lib.brt_envgrid_calc_hash(self.handle, C.byref(grid), _ptr(pos), pos.shape[0],
+                                                   _ptr(hash_out), current_stream()))
+
+    def sort(self, hash_, num_cells: int):
+        self.check(self._lib.brt_envgrid_sort(self.handle, _ptr(hash_), hash_.shape[0], num_cells,
+                                              current_stream()))
+
+    def hash_sort(self, grid: GridParams, pos, hash_out):
+        self.check(self._lib.brt_envgrid_hash_sort(self.handle, C.byref(grid), _ptr(pos), pos.shape[0],
+                                                   _ptr(hash_out), current_stream()))
+
+    def reorder(self, hash_, ins, outs):
+        n = len(ins)
+        a = (C.c_void_p * max(n, 1))(*[_ptr(t) for t in ins])
+        b = (C.c_void_p * max(n, 1))(*[_ptr(t) for t in outs])
+        self.check(self._lib.brt_envgrid_reorder(self.handle, _ptr(hash_), hash_.shape[0], n, a, b,
+                                                 current_stream()))
+
+    def cell_ranges(self, hash_, num_cells, cell_start, cell_end):
+        self.check(self._lib.brt_prox_cell_ranges(self.handle, _ptr(hash_), hash_.shape[0], num_cells,
+                                                  _ptr(cell_start), _ptr(cell_end), current_stream()))
+
+    def reorder_ranges(self, hash_, ins, outs, num_cells, cell_start, cell_end):
+        n = len(ins)
+        a = (C.c_void_p * max(n, 1))(*[_ptr(t) for t in ins])
+        b = (C.c_void_p * max(n, 1))(*[_ptr(t) for t in outs])
+        self.check(self._lib.brt_envgrid_reorder_ranges(self.handle, _ptr(hash_), hash_.shape[0], n, a, b,
+                                                        num_cells, _ptr(cell_start), _ptr(cell_end),
+                                                        current_stream()))
+
+    # ------------------------------------------------------------ Proximity3D
+    def neighbors(self, grid, prox, sorted_pos, hash_, cell_start, cell_end, lists):
+        self.check(self._lib.brt_prox_neighbors(self.handle, C.byref(grid), C.byref(prox), _ptr(sorted_pos),
+                                                _ptr(hash_), _ptr(cell_start), _ptr(cell_end),
+                                                sorted_pos.shape[0], _ptr(lists), current_stream()))
+
+    # ------------------------------------------------------------ OpenSteerWrapper
+    def steer_integrate(self, grid, prox, steer, dt, step, sorted_pos, sorted_fwd, sorted_sa, hash_, lists,
+                        pos_out, fwd_out, sa_out, wander=None, force_out=None):
+        self.check(self._lib.brt_osw_steer_integrate(
+            self.handle, C.byref(grid), C.byref(prox), C.byref(steer), float(dt), int(step),
+            sorted_pos.shape[0], _ptr(sorted_pos), _ptr(sorted_fwd), _ptr(sorted_sa), _ptr(hash_), _ptr(lists),
+            _ptr(wander), _ptr(pos_out), _ptr(fwd_out), _ptr(sa_out), _ptr(force_out), current_stream()))
+
+    def feature_update(self, prox, rate, hash_, lists, fin, fout):
+        self.check(self._lib.brt_bc_feature_update(self.handle, C.byref(prox), float(rate), hash_.shape[0],
+                                                   _ptr(hash_), _ptr(lists), _ptr(fin), _ptr(fout),
+                                                   current_stream()))
+
+    # ------------------------------------------------------------ timing
+    def phase_timing(self, enable: bool):
+        self.check(self._lib.brt_phase_timing_enable(self.handle, 1 if enable else 0))
+
+    def phase_times(self) -> tuple[dict, int]:
+        ms = (C.c_double * NUM_PHASES)()
+        steps = C.c_uint32()
+        self.check(self._lib.brt_phase_times(self.handle, ms, C.byref(steps)))
+        return {PHASE_NAMES[p]: ms[p] for p in range(NUM_PHASES)}, steps.value
+
+
+class Flock:
+    """Device buffers of one flock (the features the reference's plug-ins allocate:
+    EnvGrid3DPlugIn.h:157-173, Proximity3DPlugIn.h:167-174, OpenSteerWrapperPlugIn.h:153-178)."""
+
+    def __init__(self, ctx: Context, grid: GridParams, prox: ProxParams, steer: SteerParams, state: dict,
+                 features: np.ndarray | None = None, wander: np.ndarray | None = None, device="cuda"):
+        self.ctx = ctx
+        self.grid = grid
+        self.prox = prox
+        self.steer = steer
+        n = state["pos"].shape[0]
+        self.n = n
+        dev = torch.device(device)
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.pos = torch.from_numpy(np.ascontiguousarray(state["pos"], np.float32)).to(dev)
+        self.forward = torch.from_numpy(np.ascontiguousarray(state["forward"], np.float32)).to(dev)
+        self.smooth_accel = torch.from_numpy(np.ascontiguousarray(state["smooth_accel"], np.float32)).to(dev)
+        self.sorted_pos = torch.empty((n, 4), **f32)
+        self.sorted_forward = torch.empty((n, 4), **f32)
+        self.sorted_smooth_accel = torch.empty((n, 4), **f32)
+        u32 = dict(dtype=torch.int32, device=dev)  # uint32 bit patterns
+        self.hash = torch.empty((n, 2), **u32)
+        self.cell_start = torch.empty((grid.num_cells,), **u32)
+        self.cell_end = torch.empty((grid.num_cells,), **u32)
+        self.W = list_stride(prox.max_neighbors)
+        self.lists = torch.empty((n, self.W), **u32)
+        self.features = None
+        self.features_out = None
+        if features is not None:
+            self.features = torch.from_numpy(np.ascontiguousarray(features, np.float32)).to(dev)
+            self.features_out = torch.empty_like(self.features)
+        self.wander = None
+        if wander is not None:
+            self.wander = torch.from_numpy(np.ascontiguousarray(wander, np.float32)).to(dev)
+
+    def buffers(self) -> StepBuffers:
+        b = StepBuffers()
+        b.n = self.n
+        b.pos = _ptr(self.pos)
+        b.forward = _ptr(self.forward)
+        b.smooth_accel = _ptr(self.smooth_accel)
+        b.sorted_pos = _ptr(self.sorted_pos)
+        b.sorted_forward = _ptr(self.sorted_forward)
+        b.sorted_smooth_accel = _ptr(self.sorted_smooth_accel)
+        b.hash = _ptr(self.hash)
+        b.cell_start = _ptr(self.cell_start)
+        b.cell_end = _ptr(self.cell_end)
+        b.lists = _ptr(self.lists)
+        b.features = _ptr(self.features)
+        b.features_out = _ptr(self.features_out)
+        b.wander = _ptr(self.wander)
+        return b
+
+    # the reference's per-step functionalities, one C call each
+    def reorder_sim_data(self):
+        self.ctx.hash_sort(self.grid, self.pos, self.hash)
+        self.ctx.reorder_ranges(self.hash, [self.pos, self.forward, self.smooth_accel],
+                                [self.sorted_pos, self.sorted_forward, self.sorted_smooth_accel],
+                                self.grid.num_cells, self.cell_start, self.cell_end)
+
+    def compute_neighborhoods(self):
+        self.ctx.neighbors(self.grid, self.prox, self.sorted_pos, self.hash, self.cell_start, self.cell_end,
+                           self.lists)
+
+    def apply_steering(self, dt: float, step: int = 0, force_out=None):
+        self.ctx.steer_integrate(self.grid, self.prox, self.steer, dt, step, self.sorted_pos,
+                                 self.sorted_forward, self.sorted_smooth_accel, self.hash, self.lists,
+                                 self.pos, self.forward, self.smooth_accel, wander=self.wander,
+                                 force_out=force_out)
+
+    def update_features(self, rate: float):
+        self.ctx.feature_update(self.prox, rate, self.hash, self.lists, self.features, self.features_out)
+        self.features, self.features_out = self.features_out, self.features
+
+    def step(self, dt: float, step: int, feature_rate: float = 0.0):
+        b = self.buffers()
+        self.ctx.check(self.ctx._lib.brt_step(self.ctx.handle, C.byref(self.grid), C.byref(self.prox),
+                                              C.byref(self.steer), float(feature_rate), float(dt), int(step),
+                                              C.byref(b), current_stream()))
+        if self.features is not None and feature_rate != 0.0:
+            self.features, self.features_out = self.features_out, self.features
+
+    def state(self) -> dict:
+        torch.cuda.synchronize()
+        out = {"pos": self.pos.cpu().numpy(), "forward": self.forward.cpu().numpy(),
+               "smooth_accel": self.smooth_accel.cpu().numpy()}
+        if self.features is not None:
+            out["features"] = self.features.cpu().numpy()
+        if self.wander is not None:
+            out["wander"] = self.wander.cpu().numpy()
+        return out
+
+    def grid_outputs(self) -> dict:
+        torch.cuda.synchronize()
+        return {"hash": self.hash.cpu().numpy().view(np.uint32),
+                "cell_start": self.cell_start.cpu().numpy().view(np.uint32),
+                "cell_end": self.cell_end.cpu().numpy().view(np.uint32),
+                "lists": self.lists.cpu().numpy().view(np.uint32),
+                "sorted_pos": self.sorted_pos.cpu().numpy(),
+                "sorted_forward": self.sorted_forward.cpu().numpy(),
+                "sorted_smooth_accel": self.sorted_smooth_accel.cpu().numpy()}

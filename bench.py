@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-step simulation path on one MI355X.
+
+Workload (default): the 1,048,576-boid Gaussian-blob flock (64 blobs, sigma 10, in
+[0,400)^3, grid 100^3 of cell 4, radius 4, K 32, weights 1.0/0.8/0.6, dt 0.05).
+A step = hash + sort + reorder + cell ranges + neighbour lists + fused steering
+and integration (brt_step), inputs resident in HBM.
+
+Prints ONE JSON line: agent-updates/s over the timed steps, per-phase times and
+HBM roofline fractions (hipEvents recorded on the step's stream during a second
+pass of the same number of steps), and the CPU oracle timed on the host cores on
+a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_agent(w) -> dict:
+    """Compulsory single-touch HBM bytes per agent for each phase of brt_step.
+    D digit passes, W list words, C/N cells per agent, F = 3 reordered float4s."""
+    from behavert_amd.types import list_stride
+
+    n = w.n
+    C = w.grid.num_cells
+    bits = max(1, int(C - 1).bit_length())
+    D = (bits + 7) // 8
+    W = list_stride(w.prox.max_neighbors)
+    cpa = C / n
+    b = {
+        # prepare (empty-cell fill 8 B/cell) + hash (R pos 16, W pair 8) + sort (R 8 hist, D x (R 8 + W 8))
+        "hash_sort": 24 + (8 + 16 * D) + 8 * cpa,
+        # R pair 8 (ID + key), 3 x (gather 16 + write 16)
+        "reorder_ranges": 8 + 32 * 3,
+        # R own pos 16, W list 4W, R cell ranges 8 B/cell
+        "neighbors": 16 + 4 * W + 8 * cpa,
+        # R pos/fwd/sa 48 + list 4W + ID 4, W pos/fwd/sa 48 (neighbour gathers cache-resident)
+        "steer_integrate": 100 + 4 * W,
+        "features": 0.0,
+    }
+    if w.features is not None and w.feature_rate:
+        b["features"] = 4 + 4 * W + 32  # R ID + list, R own feature 16, W 16
+    return b
+
+
+def cpu_baseline(w, sample_steps: int) -> dict:
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    oracle.set_threads(threads)
+    hf = oracle.HostFlock(w.grid, w.prox, w.state, features=w.features, wander=w.wander)
+    hf.step(w.steer, w.dt, 0, w.feature_rate)  # first touch of the buffers, untimed
+    t0 = time.perf_counter()
+    for i in range(sample_steps):
+        hf.step(w.steer, w.dt, 1 + i, w.feature_rate)
+    dt = time.perf_counter() - t0
+    return {"value": w.n * sample_steps / dt, "unit": "agent-updates/s", "cores": oracle.num_threads(),
+            "kind": "port",
+            "sample": f"{sample_steps} steps of the {w.name} workload ({w.n} agents) after 1 untimed step, "
+                      f"FP32 C++ oracle (hash, LSD radix sort, gather, ranges, 27-cell lists, steering), OpenMP",
+            "ms_per_step": 1e3 * dt / sample_steps}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="blobs_1m")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="CPU baseline sample steps (0 disables)")
+    ap.add_argument("--no-phases", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU: the step path has no multi-GPU form "
+                                   "(see DESIGN.md, 'How the path would shard')", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from behavert_amd import workloads as W
+    from behavert_amd.engine import Context, Flock
+
+    w = W.WORKLOADS[args.workload]()
+    ctx = Context(0)
+    fl = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander)
+
+    def run(first, count):
+        for i in range(first, first + count):
+            if w.moving_targets:
+                fl.steer.set_targets(w.targets_at(i))
+            fl.step(w.dt, i, w.feature_rate)
+
+    run(0, args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.warmup, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = w.n * args.steps / elapsed
+
+    abytes = algorithmic_bytes_per_agent(w)
+    total_b = sum(abytes.values())
+    phases = {}
+    roofline = None
+    if not args.no_phases:
+        ctx.phase_timing(True)
+        run(args.warmup + args.steps, args.steps)
+        ms, nsteps = ctx.phase_times()
+        ctx.phase_timing(False)
+        for k, v in ms.items():
+            avg = v / max(nsteps, 1)
+            gbs = (abytes[k] * w.n / 1e9) / (avg / 1e3) if avg > 0 and abytes[k] > 0 else 0.0
+            phases[k] = {"ms": round(avg, 5), "bytes_per_agent": round(abytes[k], 2), "GB/s": round(gbs, 1),
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        dom = max((k for k in phases if abytes[k] > 0), key=lambda k: phases[k]["ms"])
+        roofline = {"bound": "hbm", "kernel": {"neighbors": "k_neighbors", "steer_integrate": "k_steer",
+                                               "hash_sort": "k_hash_hist+k_onesweep",
+                                               "reorder_ranges": "k_reorder_ranges",
+                                               "features": "k_features"}[dom],
+                    "phase": dom, "achieved": phases[dom]["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": phases[dom]["frac"], "traffic": None,
+                    "bytes_per_agent": round(abytes[dom], 2), "agents_per_launch": w.n}
+        g_keys = ("hash_sort", "reorder_ranges", "neighbors")
+        g_ms = sum(phases[k]["ms"] for k in g_keys)
+        g_b = sum(abytes[k] for k in g_keys) * w.n
+        phases["grid_neighbor_total"] = {"ms": round(g_ms, 5), "GB/s": round(g_b / 1e9 / (g_ms / 1e3), 1),
+                                         "frac": round(g_b / 1e9 / (g_ms / 1e3) / HBM_PEAK_GBS, 4)}
+    step_gbs = total_b * w.n / 1e9 / (ms_per_step / 1e3)
+
+    cpu = cpu_baseline(w, args.cpu_steps) if args.cpu_steps > 0 else None
+    line = {
+        "metric": "agent-updates/s at 1M boids (% HBM peak); fwd+adjoint vs plain step time",
+        "value": value,
+        "unit": "agent-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": w.describe(),
+        "hbm_frac_step": round(step_gbs / HBM_PEAK_GBS, 4),
+        "roofline": roofline,
+        "phases": phases,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

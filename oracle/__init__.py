@@ -1,0 +1,251 @@
+"""CPU oracle for the BehaveRT per-step path (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may
+import this package, and only as the checker or as the timed CPU baseline; the
+product (behavert_amd) never does.
+
+numpy wrappers over oracle/liboracle.so (built from oracle/brt_oracle.cpp by
+oracle/build.sh).  See brt_oracle.cpp for the reference file:line each function
+restates.  Parity pins (tests/test_oracle_*.py):
+  calc_hash ........ closed forms (face points p = 4k, clamping, NaN, mul24)
+  sort_radix ....... std::stable_sort, bijection/order invariants
+  cell_ranges ...... closed form from sorted keys
+  neighbors ........ O(N^2) brute force in canonical order, KATs at d = r
+  steer_f32 ........ FP64 evaluation, hand-derived 2/3-agent KATs
+  philox ........... Random123 known-answer vectors
+  feature_update ... closed form (uniform features are a fixed point)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from behavert_amd.types import GridParams, ProxParams, SteerParams, StepBuffers, list_stride
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["bash", os.path.join(HERE, "build.sh")], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        U = C.c_uint32
+        F = C.c_float
+        GP, PP, SP = C.POINTER(GridParams), C.POINTER(ProxParams), C.POINTER(SteerParams)
+        sig = {
+            "orc_set_threads": (None, [C.c_int]),
+            "orc_num_threads": (C.c_int, []),
+            "orc_philox4x32_10": (None, [P, P, P]),
+            "orc_grid_setup": (None, [P, P, P, U, GP]),
+            "orc_calc_grid_pos": (None, [GP, P, U, P]),
+            "orc_calc_hash": (None, [GP, P, U, P]),
+            "orc_sort_stable": (None, [P, U]),
+            "orc_sort_radix": (None, [P, U, U]),
+            "orc_reorder": (None, [P, U, P, P]),
+            "orc_cell_ranges": (None, [P, U, U, P, P]),
+            "orc_neighbors": (None, [GP, PP, P, P, P, P, U, P]),
+            "orc_neighbors_brute": (None, [GP, PP, P, P, U, P]),
+            "orc_steer_f32": (None, [GP, PP, SP, F, U, U, P, P, P, P, P, P, P, P, P, P]),
+            "orc_steer_f64": (None, [GP, PP, SP, F, U, U, P, P, P, P, P, P, P, P, P, P]),
+            "orc_feature_update_f32": (None, [PP, F, U, P, P, P, P]),
+            "orc_feature_update_f64": (None, [PP, F, U, P, P, P, P]),
+            "orc_step_f32": (None, [GP, PP, SP, F, F, U, C.POINTER(StepBuffers), P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def grid_setup(center, radius, cell, max_bodies=0xFFFFFFFF) -> GridParams:
+    g = GridParams()
+    c = np.ascontiguousarray(center, np.float32)
+    r = np.ascontiguousarray(radius, np.float32)
+    s = np.ascontiguousarray(cell, np.float32)
+    lib().orc_grid_setup(_p(c), _p(r), _p(s), int(max_bodies), C.byref(g))
+    return g
+
+
+def calc_grid_pos(grid, pos) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float32)
+    out = np.zeros((len(pos), 3), np.int32)
+    lib().orc_calc_grid_pos(C.byref(grid), _p(pos), len(pos), _p(out))
+    return out
+
+
+def calc_hash(grid, pos) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float32)
+    h = np.zeros((len(pos), 2), np.uint32)
+    lib().orc_calc_hash(C.byref(grid), _p(pos), len(pos), _p(h))
+    return h
+
+
+def sort_stable(h) -> np.ndarray:
+    h = np.array(h, np.uint32, copy=True, order="C")
+    lib().orc_sort_stable(_p(h), len(h))
+    return h
+
+
+def sort_radix(h, num_cells) -> np.ndarray:
+    h = np.array(h, np.uint32, copy=True, order="C")
+    lib().orc_sort_radix(_p(h), len(h), int(num_cells))
+    return h
+
+
+def reorder(h, arr) -> np.ndarray:
+    arr = np.ascontiguousarray(arr, np.float32)
+    out = np.zeros_like(arr)
+    lib().orc_reorder(_p(np.ascontiguousarray(h, np.uint32)), len(h), _p(arr), _p(out))
+    return out
+
+
+def cell_ranges(h, num_cells):
+    s = np.zeros(num_cells, np.uint32)
+    e = np.zeros(num_cells, np.uint32)
+    lib().orc_cell_ranges(_p(np.ascontiguousarray(h, np.uint32)), len(h), int(num_cells), _p(s), _p(e))
+    return s, e
+
+
+def neighbors(grid, prox, spos, h, start, end) -> np.ndarray:
+    n = len(spos)
+    lists = np.zeros((n, list_stride(prox.max_neighbors)), np.uint32)
+    lib().orc_neighbors(C.byref(grid), C.byref(prox), _p(np.ascontiguousarray(spos, np.float32)),
+                        _p(np.ascontiguousarray(h, np.uint32)), _p(np.ascontiguousarray(start, np.uint32)),
+                        _p(np.ascontiguousarray(end, np.uint32)), n, _p(lists))
+    return lists
+
+
+def neighbors_brute(grid, prox, spos, h) -> np.ndarray:
+    n = len(spos)
+    lists = np.zeros((n, list_stride(prox.max_neighbors)), np.uint32)
+    lib().orc_neighbors_brute(C.byref(grid), C.byref(prox), _p(np.ascontiguousarray(spos, np.float32)),
+                              _p(np.ascontiguousarray(h, np.uint32)), n, _p(lists))
+    return lists
+
+
+def _steer(fn, dtype, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None):
+    n = len(spos)
+    spos = np.ascontiguousarray(spos, dtype)
+    sfwd = np.ascontiguousarray(sfwd, dtype)
+    ssa = np.ascontiguousarray(ssa, dtype)
+    pos = np.zeros((n, 4), dtype)
+    fwd = np.zeros((n, 4), dtype)
+    sa = np.zeros((n, 4), dtype)
+    force = np.zeros((n, 4), dtype)
+    w = None if wander is None else np.array(wander, dtype, copy=True, order="C")
+    fn(C.byref(grid), C.byref(prox), C.byref(steer), float(dt), int(step), n, _p(spos), _p(sfwd), _p(ssa),
+       _p(np.ascontiguousarray(h, np.uint32)), _p(np.ascontiguousarray(lists, np.uint32)), _p(w), _p(pos),
+       _p(fwd), _p(sa), _p(force))
+    return {"pos": pos, "forward": fwd, "smooth_accel": sa, "force": force, "wander": w}
+
+
+def steer_f32(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None):
+    return _steer(lib().orc_steer_f32, np.float32, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander)
+
+
+def steer_f64(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None):
+    return _steer(lib().orc_steer_f64, np.float64, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander)
+
+
+def feature_update_f32(prox, rate, h, lists, fin):
+    fin = np.ascontiguousarray(fin, np.float32)
+    out = np.zeros_like(fin)
+    lib().orc_feature_update_f32(C.byref(prox), float(rate), len(h), _p(np.ascontiguousarray(h, np.uint32)),
+                                 _p(np.ascontiguousarray(lists, np.uint32)), _p(fin), _p(out))
+    return out
+
+
+def feature_update_f64(prox, rate, h, lists, fin):
+    fin = np.ascontiguousarray(fin, np.float64)
+    out = np.zeros_like(fin)
+    lib().orc_feature_update_f64(C.byref(prox), float(rate), len(h), _p(np.ascontiguousarray(h, np.uint32)),
+                                 _p(np.ascontiguousarray(lists, np.uint32)), _p(fin), _p(out))
+    return out
+
+
+class HostFlock:
+    """Host buffers for orc_step_f32 (same layout as the device flock)."""
+
+    def __init__(self, grid, prox, state, features=None, wander=None):
+        n = len(state["pos"])
+        self.n = n
+        self.grid, self.prox = grid, prox
+        self.pos = np.array(state["pos"], np.float32, copy=True, order="C")
+        self.forward = np.array(state["forward"], np.float32, copy=True, order="C")
+        self.smooth_accel = np.array(state["smooth_accel"], np.float32, copy=True, order="C")
+        self.sorted_pos = np.zeros((n, 4), np.float32)
+        self.sorted_forward = np.zeros((n, 4), np.float32)
+        self.sorted_smooth_accel = np.zeros((n, 4), np.float32)
+        self.hash = np.zeros((n, 2), np.uint32)
+        self.cell_start = np.zeros(grid.num_cells, np.uint32)
+        self.cell_end = np.zeros(grid.num_cells, np.uint32)
+        self.lists = np.zeros((n, list_stride(prox.max_neighbors)), np.uint32)
+        self.features = None if features is None else np.array(features, np.float32, copy=True, order="C")
+        self.features_out = None if features is None else np.zeros_like(self.features)
+        self.wander = None if wander is None else np.array(wander, np.float32, copy=True, order="C")
+        self.phase_s = np.zeros(5, np.float64)
+
+    def step(self, steer, dt, step, feature_rate=0.0):
+        b = StepBuffers()
+        b.n = self.n
+        for name in ("pos", "forward", "smooth_accel", "sorted_pos", "sorted_forward", "sorted_smooth_accel",
+                     "hash", "cell_start", "cell_end", "lists", "features", "features_out", "wander"):
+            setattr(b, name, _p(getattr(self, name)))
+        lib().orc_step_f32(C.byref(self.grid), C.byref(self.prox), C.byref(steer), float(feature_rate),
+                           float(dt), int(step), C.byref(b), _p(self.phase_s))
+        if self.features is not None and feature_rate != 0.0:
+            self.features, self.features_out = self.features_out, self.features
+
+    def state(self):
+        out = {"pos": self.pos.copy(), "forward": self.forward.copy(), "smooth_accel": self.smooth_accel.copy()}
+        if self.features is not None:
+            out["features"] = self.features.copy()
+        if self.wander is not None:
+            out["wander"] = self.wander.copy()
+        return out
+
+
+def step_f64_fixed(grid, prox, steer, dt, step, state64, h, lists, wander64=None):
+    """One FP64 step from an FP64 state (agent-ID order) with the sort and the
+    neighbour lists fixed to (h, lists) (e.g. those the FP32 step produced)."""
+    ids = h[:, 1].astype(np.int64)
+    spos = state64["pos"][ids]
+    sfwd = state64["forward"][ids]
+    ssa = state64["smooth_accel"][ids]
+    out = steer_f64(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander64)
+    return out

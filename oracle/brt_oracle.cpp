@@ -1,0 +1,778 @@
+// brt_oracle.cpp -- CPU oracle for the BehaveRT per-step simulation path.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load this library, and only as the checker
+// (or as the timed CPU baseline).  The product path never calls it.
+//
+// It restates, on the CPU, the algorithm of the reference's per-step path:
+//   calcGridPos / calcGridHash ....... PlugIns/EnvGrid3D/envgrid3d_resources.cu:60-84
+//   genCalcHashD ..................... PlugIns/EnvGrid3D/envgrid3d_kernel.cu:55-68
+//   RadixSort (stable, key/value) .... PlugIns/EnvGrid3D/radixsort.cu:50-80
+//   reorderDataFloat4 ................ PlugIns/EnvGrid3D/envgrid3d_kernel.cu:110-122
+//   genericFindCellStartD ............ PlugIns/Proximity3D/Proximity3D_kernel.cu:54-84
+//   genericCalcNeighgborhoodD ........ PlugIns/Proximity3D/Proximity3D_kernel.cu:89-125
+//   checkForNeighborsInCell/Neighbor . PlugIns/Proximity3D/Proximity3D_resources.cu:59-172
+//   getNeighborsList (W layout) ...... PlugIns/Proximity3D/Proximity3D_resources.cu:175-198
+//   inBoidNeighborhood, calcSeparation/Cohesion/Alignment, xxxsteerForSeek,
+//   applySteeringForceSingle ......... PlugIns/OpenSteerWrapper/OpenSteerWrapper_resources.cu:67-379
+//   steering kernels + integrator .... PlugIns/OpenSteerWrapper/OpenSteerWrapper_kernel.cu:62-333
+//   truncateLength, adjustRawSteeringForce, vecLimitDeviationAngleUtility,
+//   blendIntoAccumulator3, interpolate Core/common_resources.cu:140-271
+//   step order ....................... Samples/OpensteerDemo/plugins/BehaveRT_Flocking.cpp:177-196
+//   steerForWander / scalarRandomWalk  Samples/OpensteerDemo/include/OpenSteer/SteerLibrary.h:319-331,
+//                                      Samples/OpensteerDemo/include/OpenSteer/Utilities.h:158-167
+//
+// Two precisions of the floating-point part share one template:
+//   * FP32 mirror (R = float): the reference's operation order, IEEE single
+//     precision, compiled with -ffp-contract=off (no FMA), correctly rounded
+//     sqrt and division.  It also defines the integer outputs (keys, stable
+//     permutation, cell ranges, neighbour lists) bit for bit.
+//   * FP64 (R = double): the same function evaluated in double precision from
+//     the same FP32 inputs and constants; the reference point for the float
+//     tolerances.
+//
+// Build: oracle/build.sh (g++ -O2 -ffp-contract=off -fopenmp).
+//
+// Choices this oracle fixes where the reference leaves the last bit open
+// (cutil_math is not in the reference tree):
+//   normalize(v)  = v * (1 / sqrt(dot(v,v)))   (reference: v * rsqrtf(dot))
+//   v / s         = (v.x / s, v.y / s, v.z / s)
+//   powf(x, 20)   = ((x^2)^2 * x)^2)^2          (repeated squaring)
+// Each is compared against the FP64 evaluation by the tests, not bit-pinned to
+// the reference GPU (which cannot run here).
+
+#include "../include/behavert_amd.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ------------------------------------------------------------------ vectors
+
+template <typename R>
+struct V3 {
+    R x, y, z;
+};
+
+template <typename R>
+inline V3<R> v3(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <typename R>
+inline V3<R> operator+(V3<R> a, V3<R> b) { return v3<R>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <typename R>
+inline V3<R> operator-(V3<R> a, V3<R> b) { return v3<R>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename R>
+inline V3<R> operator*(V3<R> a, R s) { return v3<R>(a.x * s, a.y * s, a.z * s); }
+template <typename R>
+inline V3<R> operator/(V3<R> a, R s) { return v3<R>(a.x / s, a.y / s, a.z / s); }
+
+inline float rsqrt_sqrt(float x) { return sqrtf(x); }
+inline double rsqrt_sqrt(double x) { return std::sqrt(x); }
+
+// cutil_math dot: a.x*b.x + a.y*b.y + a.z*b.z, evaluated left to right.
+template <typename R>
+inline R dot(V3<R> a, V3<R> b) {
+    R xx = a.x * b.x;
+    R yy = a.y * b.y;
+    R zz = a.z * b.z;
+    return (xx + yy) + zz;
+}
+template <typename R>
+inline R length(V3<R> v) { return rsqrt_sqrt(dot(v, v)); }
+template <typename R>
+inline V3<R> normalize(V3<R> v) {
+    R inv = R(1) / rsqrt_sqrt(dot(v, v));
+    return v * inv;
+}
+template <typename R>
+inline V3<R> cross(V3<R> a, V3<R> b) {
+    return v3<R>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <typename R>
+inline R clampr(R x, R lo, R hi) {  // cutil clamp = fminf(fmaxf(x, lo), hi)
+    R t = x < lo ? lo : x;
+    return t > hi ? hi : t;
+}
+
+// Core/common_resources.cu:260-271
+template <typename R>
+inline V3<R> truncateLength(V3<R> v, R maxLength) {
+    R len = length(v);
+    if (len <= maxLength) return v;
+    return v * (maxLength / len);
+}
+
+template <typename R>
+inline R pow20(R x) {
+    R x2 = x * x;
+    R x4 = x2 * x2;
+    R x5 = x4 * x;
+    R x10 = x5 * x5;
+    return x10 * x10;
+}
+
+// Core/common_resources.cu:186-228 (insideOrOutside == true)
+template <typename R>
+inline V3<R> vecLimitDeviationAngle(V3<R> source, R cosineOfConeAngle, V3<R> basis) {
+    R sourceLength = length(source);
+    if (sourceLength == R(0)) return source;
+    V3<R> direction = source / sourceLength;
+    R cosineOfSourceAngle = dot(direction, basis);
+    if (cosineOfSourceAngle >= cosineOfConeAngle) return source;
+    R projection = dot(source, basis);
+    V3<R> perp = source - basis * projection;
+    V3<R> unitPerp = normalize(perp);
+    R perpDist = rsqrt_sqrt(R(1) - cosineOfConeAngle * cosineOfConeAngle);
+    V3<R> c0 = basis * cosineOfConeAngle;
+    V3<R> c1 = unitPerp * perpDist;
+    return (c0 + c1) * sourceLength;
+}
+
+// Core/common_resources.cu:234-254
+template <typename R>
+inline V3<R> adjustRawSteeringForce(V3<R> force, R speed, V3<R> forward, R maxSpeed) {
+    R maxAdjustedSpeed = R(0.2f) * maxSpeed;
+    if (speed > maxAdjustedSpeed || (force.x == R(0) && force.y == R(0) && force.z == R(0)))
+        return force;
+    R range = speed / maxAdjustedSpeed;
+    R cosine = R(1) + (R(-2) * pow20(range));  // interpolate(p, 1, -1)
+    return vecLimitDeviationAngle(force, cosine, forward);
+}
+
+// OpenSteerWrapper_resources.cu:67-92
+template <typename R>
+inline bool inBoidNeighborhood(V3<R> myPos, V3<R> otherPos, R minDist, R maxDist,
+                               R cosMaxAngle, V3<R> forward) {
+    V3<R> offset = otherPos - myPos;
+    R dist = length(offset);
+    if (dist < minDist) return true;
+    if (dist > maxDist) return false;
+    R cosAngle = dot(forward, normalize(offset));
+    return cosAngle > cosMaxAngle;
+}
+
+// OpenSteerWrapper_resources.cu:294-306
+template <typename R>
+inline V3<R> steerForSeek(V3<R> myPos, R mySpeed, V3<R> myForward, V3<R> target, R maxSpeed) {
+    V3<R> offset = target - myPos;
+    V3<R> desired = truncateLength(offset, maxSpeed);
+    V3<R> myVel = myForward * mySpeed;
+    return desired - myVel;
+}
+
+// ------------------------------------------------------------------ grid math
+
+// float -> int conversion as CUDA's cvt.rzi.s32.f32 does it (NaN -> 0, saturating).
+inline int f2i(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT_MAX;
+    if (x <= -2147483648.0f) return INT_MIN;
+    return (int)x;
+}
+
+// __mul24: low 32 bits of the product of the sign-extended low 24 bits.
+inline int mul24(int a, int b) {
+    int64_t sa = (int64_t)(int32_t)((uint32_t)a << 8) >> 8;
+    int64_t sb = (int64_t)(int32_t)((uint32_t)b << 8) >> 8;
+    return (int)(uint32_t)(uint64_t)(sa * sb);
+}
+
+struct I3 {
+    int x, y, z;
+};
+
+// envgrid3d_resources.cu:60-67
+inline I3 calcGridPos(const float* p, const brt_grid_params& g) {
+    I3 r;
+    r.x = f2i(floorf((p[0] - g.world_origin[0]) / g.cell_size[0]));
+    r.y = f2i(floorf((p[1] - g.world_origin[1]) / g.cell_size[1]));
+    r.z = f2i(floorf((p[2] - g.world_origin[2]) / g.cell_size[2]));
+    return r;
+}
+
+// envgrid3d_resources.cu:75-84.  max(0, min(int, uint)) resolves to CUDA's
+// mixed-sign overloads, which compare as unsigned: a negative coordinate maps
+// to gridSize-1, a coordinate above the grid to gridSize-1.
+inline uint32_t clampAxis(int v, uint32_t size) {
+    uint32_t u = (uint32_t)v;
+    uint32_t hi = size - 1u;
+    return u < hi ? u : hi;
+}
+
+inline uint32_t calcGridHash(I3 gp, const brt_grid_params& g) {
+    int x = (int)clampAxis(gp.x, g.grid_size[0]);
+    int y = (int)clampAxis(gp.y, g.grid_size[1]);
+    int z = (int)clampAxis(gp.z, g.grid_size[2]);
+    int gx = (int)g.grid_size[0], gy = (int)g.grid_size[1];
+    return (uint32_t)(mul24(mul24(z, gy), gx) + mul24(y, gx) + x);
+}
+
+inline bool inGrid(I3 c, const brt_grid_params& g) {
+    return c.x >= 0 && c.x <= (int)g.grid_size[0] - 1 && c.y >= 0 &&
+           c.y <= (int)g.grid_size[1] - 1 && c.z >= 0 && c.z <= (int)g.grid_size[2] - 1;
+}
+
+inline uint32_t listStride(uint32_t K) { return 4u * (K / 4u + 1u); }
+
+// ------------------------------------------------------------------ Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11.
+inline void philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+inline float u24(uint32_t r) { return (float)(r >> 8) * (1.0f / 16777216.0f); }
+
+int g_threads = 0;
+
+inline void set_threads() {
+#ifdef _OPENMP
+    if (g_threads > 0) omp_set_num_threads(g_threads);
+#endif
+}
+
+// ------------------------------------------------------------------ steering (one agent)
+
+template <typename R>
+struct SteerArrays {
+    const R* pos;    // sorted, 4 per agent
+    const R* fwd;    // sorted
+    const R* sa;     // sorted
+    const uint32_t* hash;
+    const uint32_t* lists;
+    uint32_t W;
+    R* wander;       // by ID, 2 per agent (in/out), or null
+    R* pos_out;      // by ID
+    R* fwd_out;
+    R* sa_out;
+    R* force_out;    // sorted, optional
+};
+
+template <typename R>
+inline V3<R> ld3(const R* a, uint32_t i) { return v3<R>(a[4 * i], a[4 * i + 1], a[4 * i + 2]); }
+
+template <typename R>
+void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& sp, R dt,
+                 uint32_t step, const SteerArrays<R>& A) {
+    const V3<R> myPos = ld3(A.pos, i);
+    const V3<R> myFwd = ld3(A.fwd, i);
+    R mySpeed = A.fwd[4 * i + 3];
+    V3<R> mySa = ld3(A.sa, i);
+    const R saW = A.sa[4 * i + 3];
+    const uint32_t id = A.hash[2 * i + 1];
+    const uint32_t* L = A.lists + (size_t)i * A.W;
+    const uint32_t count = L[0];
+    const R minR = R(sp.sensing_min_radius);
+
+    V3<R> force = v3<R>(0, 0, 0);
+
+    // genericSeparationD (OpenSteerWrapper_kernel.cu:86-105), calcSeparation (:166-211)
+    if (sp.separation[0] != 0.0f) {
+        V3<R> s = v3<R>(0, 0, 0);
+        uint32_t nb = 0;
+        for (uint32_t t = 0; t < count; ++t) {
+            uint32_t j = L[1 + t];
+            V3<R> other = ld3(A.pos, j);
+            if (inBoidNeighborhood(myPos, other, minR, R(sp.separation[1]), R(sp.separation[2]), myFwd)) {
+                V3<R> offset = other - myPos;
+                R d2 = dot(offset, offset);
+                s = s + offset / (-d2);
+                ++nb;
+            }
+        }
+        if (nb > 0) s = normalize(s / R(nb));
+        force = s * R(sp.separation[0]) + force;
+    }
+    // genericAlignmentD (:158-178), calcAlignment (:253-290)
+    if (sp.alignment[0] != 0.0f) {
+        V3<R> s = v3<R>(0, 0, 0);
+        uint32_t nb = 0;
+        for (uint32_t t = 0; t < count; ++t) {
+            uint32_t j = L[1 + t];
+            V3<R> other = ld3(A.pos, j);
+            if (inBoidNeighborhood(myPos, other, minR, R(sp.alignment[1]), R(sp.alignment[2]), myFwd)) {
+                s = s + ld3(A.fwd, j);
+                ++nb;
+            }
+        }
+        if (nb > 0) s = normalize(s / R(nb) - myFwd);
+        force = s * R(sp.alignment[0]) + force;
+    }
+    // genericCohesionD (:62-81), calcCohesion (:216-249)
+    if (sp.cohesion[0] != 0.0f) {
+        V3<R> s = v3<R>(0, 0, 0);
+        uint32_t nb = 0;
+        for (uint32_t t = 0; t < count; ++t) {
+            uint32_t j = L[1 + t];
+            V3<R> other = ld3(A.pos, j);
+            if (inBoidNeighborhood(myPos, other, minR, R(sp.cohesion[1]), R(sp.cohesion[2]), myFwd)) {
+                s = s + other;
+                ++nb;
+            }
+        }
+        if (nb > 0) s = normalize(s / R(nb) - myPos);
+        force = s * R(sp.cohesion[0]) + force;
+    }
+    // multi-target seek (build-defined; xxxsteerForSeek toward targets[id % T])
+    if (sp.num_targets > 0 && sp.target_weight != 0.0f) {
+        const float* t = sp.targets[id % sp.num_targets];
+        V3<R> tgt = v3<R>(R(t[0]), R(t[1]), R(t[2]));
+        V3<R> s = steerForSeek(myPos, mySpeed, myFwd, tgt, R(sp.max_speed));
+        force = s * R(sp.target_weight) + force;
+    }
+    // wander (build-defined; OpenSteer steerForWander with a counter-based RNG)
+    if (sp.wander_weight != 0.0f && A.wander) {
+        uint32_t ctr[4] = {step, 0u, 0u, 0u};
+        uint32_t key[2] = {sp.wander_seed, id};
+        uint32_t r[4];
+        philox4x32_10(ctr, key, r);
+        R u0 = R(u24(r[0])), u1 = R(u24(r[1]));
+        R walk = R(12) * dt;
+        R ws = A.wander[2 * id], wu = A.wander[2 * id + 1];
+        ws = ws + ((u0 * R(2) - R(1)) * walk);
+        ws = ws < R(-1) ? R(-1) : (ws > R(1) ? R(1) : ws);
+        wu = wu + ((u1 * R(2) - R(1)) * walk);
+        wu = wu < R(-1) ? R(-1) : (wu > R(1) ? R(1) : wu);
+        A.wander[2 * id] = ws;
+        A.wander[2 * id + 1] = wu;
+        V3<R> c = cross(v3<R>(0, 1, 0), myFwd);
+        V3<R> side = dot(c, c) == R(0) ? v3<R>(1, 0, 0) : normalize(c);
+        V3<R> up = cross(myFwd, side);
+        V3<R> s = side * ws + up * wu;
+        force = s * R(sp.wander_weight) + force;
+    }
+    // genericSeekingWorldCenterD<false> (OpenSteerWrapper_kernel.cu:219-259)
+    if (sp.seek_world_center) {
+        const R cx = R(g.world_center[0]), cy = R(g.world_center[1]), cz = R(g.world_center[2]);
+        const R rx = R(g.world_radius[0]), ry = R(g.world_radius[1]), rz = R(g.world_radius[2]);
+        bool inside = (myPos.x > cx - rx) && (myPos.x < cx + rx) && (myPos.y > cy - ry) &&
+                      (myPos.y < cy + ry) && (myPos.z > cz - rz) && (myPos.z < cz + rz);
+        V3<R> s = inside ? v3<R>(0, 0, 0)
+                         : steerForSeek(myPos, mySpeed, myFwd, v3<R>(cx, cy, cz), R(sp.max_speed));
+        force = s + force;
+    }
+    if (A.force_out) {
+        A.force_out[4 * i] = force.x;
+        A.force_out[4 * i + 1] = force.y;
+        A.force_out[4 * i + 2] = force.z;
+        A.force_out[4 * i + 3] = R(0);
+    }
+
+    // genericApplySteeringForceD (OpenSteerWrapper_kernel.cu:265-333) +
+    // applySteeringForceSingle (OpenSteerWrapper_resources.cu:326-379)
+    if (sp.use_2d_projection) force.y = R(0);
+    const R maxSpeed = R(sp.max_speed);
+    V3<R> adjusted = adjustRawSteeringForce(force, mySpeed, myFwd, maxSpeed);
+    V3<R> clipped = truncateLength(adjusted, R(sp.max_force));
+    V3<R> newAccel = clipped / R(sp.mass);
+    if (dt > R(0)) {
+        R smoothRate = clampr(R(9) * dt, R(0.15f), R(0.4f));
+        smoothRate = clampr(smoothRate, R(0), R(1));
+        mySa = mySa + (newAccel - mySa) * smoothRate;
+    }
+    V3<R> newVel = myFwd * mySpeed + mySa * dt;
+    newVel = truncateLength(newVel, maxSpeed);
+    mySpeed = length(newVel);
+    V3<R> newPos = myPos + newVel * dt;
+    if (sp.use_2d_projection) newVel.y = R(0);
+    V3<R> newFwd = normalize(newVel);
+    if (sp.use_2d_projection) newPos.y = R(0.5f);
+
+    R* P = A.pos_out + 4 * (size_t)id;
+    P[0] = newPos.x; P[1] = newPos.y; P[2] = newPos.z; P[3] = R(1);
+    R* S = A.sa_out + 4 * (size_t)id;
+    S[0] = mySa.x; S[1] = mySa.y; S[2] = mySa.z; S[3] = saW;
+    R* F = A.fwd_out + 4 * (size_t)id;
+    F[0] = newFwd.x; F[1] = newFwd.y; F[2] = newFwd.z; F[3] = mySpeed;
+}
+
+template <typename R>
+void feature_agent(uint32_t i, R rate, const uint32_t* hash, const uint32_t* lists, uint32_t W,
+                   const R* fin, R* fout) {
+    const uint32_t id = hash[2 * i + 1];
+    const uint32_t* L = lists + (size_t)i * W;
+    const uint32_t count = L[0];
+    const R* f = fin + 4 * (size_t)id;
+    R* o = fout + 4 * (size_t)id;
+    if (count == 0) {
+        for (int c = 0; c < 4; ++c) o[c] = f[c];
+        return;
+    }
+    R s[4] = {0, 0, 0, 0};
+    for (uint32_t t = 0; t < count; ++t) {
+        uint32_t jid = hash[2 * L[1 + t] + 1];
+        const R* q = fin + 4 * (size_t)jid;
+        for (int c = 0; c < 4; ++c) s[c] = s[c] + q[c];
+    }
+    for (int c = 0; c < 4; ++c) {
+        R mean = s[c] / R(count);
+        o[c] = f[c] + (mean - f[c]) * rate;
+    }
+}
+
+// ------------------------------------------------------------------ sort
+
+void radix_sort_pairs(uint32_t* pairs, uint32_t n, uint32_t num_cells, std::vector<uint32_t>& tmp) {
+    uint32_t bits = 0;
+    while (bits < 32 && (num_cells - 1u) >> bits) ++bits;
+    uint32_t passes = (bits + 7) / 8;
+    if (passes == 0 || n < 2) return;
+    tmp.resize(2 * (size_t)n);
+    uint32_t* src = pairs;
+    uint32_t* dst = tmp.data();
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    std::vector<uint32_t> hist((size_t)nt * 256);
+    for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t shift = 8 * p;
+        std::fill(hist.begin(), hist.end(), 0u);
+#pragma omp parallel num_threads(nt)
+        {
+            int t = 0, T = 1;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+            T = omp_get_num_threads();
+#endif
+            size_t lo = (size_t)n * t / T, hi = (size_t)n * (t + 1) / T;
+            uint32_t* h = &hist[(size_t)t * 256];
+            for (size_t i = lo; i < hi; ++i) h[(src[2 * i] >> shift) & 255u]++;
+#pragma omp barrier
+#pragma omp single
+            {
+                uint32_t run = 0;
+                for (uint32_t d = 0; d < 256; ++d)
+                    for (int u = 0; u < T; ++u) {
+                        uint32_t c = hist[(size_t)u * 256 + d];
+                        hist[(size_t)u * 256 + d] = run;
+                        run += c;
+                    }
+            }
+            for (size_t i = lo; i < hi; ++i) {
+                uint32_t d = (src[2 * i] >> shift) & 255u;
+                uint32_t o = h[d]++;
+                dst[2 * (size_t)o] = src[2 * i];
+                dst[2 * (size_t)o + 1] = src[2 * i + 1];
+            }
+        }
+        std::swap(src, dst);
+    }
+    if (src != pairs) std::memcpy(pairs, src, sizeof(uint32_t) * 2 * (size_t)n);
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+void orc_set_threads(int threads) { g_threads = threads; }
+
+int orc_num_threads(void) {
+    set_threads();
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    philox4x32_10(ctr, key, out);
+}
+
+// EnvGrid3DPlugIn::setupGrid (PlugIns/EnvGrid3D/include/EnvGrid3DPlugIn.h:478-508)
+void orc_grid_setup(const float center[3], const float radius[3], const float cell[3],
+                    uint32_t max_bodies_per_cell, brt_grid_params* out) {
+    std::memset(out, 0, sizeof(*out));
+    for (int a = 0; a < 3; ++a) {
+        float aug = radius[a] + cell[a] * 2.0f;
+        out->grid_size[a] = (uint32_t)(aug * 2.0f / cell[a]);
+        out->world_origin[a] = center[a] - aug;
+        out->cell_size[a] = cell[a];
+        out->world_center[a] = center[a];
+        out->world_radius[a] = radius[a];
+    }
+    out->num_cells = out->grid_size[0] * out->grid_size[1] * out->grid_size[2];
+    out->max_bodies_per_cell = max_bodies_per_cell;
+}
+
+void orc_calc_grid_pos(const brt_grid_params* g, const float* pos, uint32_t n, int32_t* out3) {
+    for (uint32_t i = 0; i < n; ++i) {
+        I3 c = calcGridPos(pos + 4 * (size_t)i, *g);
+        out3[3 * i] = c.x;
+        out3[3 * i + 1] = c.y;
+        out3[3 * i + 2] = c.z;
+    }
+}
+
+// genCalcHashD
+void orc_calc_hash(const brt_grid_params* g, const float* pos, uint32_t n, uint32_t* hash) {
+    set_threads();
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        hash[2 * i] = calcGridHash(calcGridPos(pos + 4 * i, *g), *g);
+        hash[2 * i + 1] = (uint32_t)i;
+    }
+}
+
+// Textbook stable sort by key (the semantics of the reference's LSD radix sort).
+void orc_sort_stable(uint32_t* hash, uint32_t n) {
+    std::vector<uint64_t> v(n);
+    for (uint32_t i = 0; i < n; ++i) v[i] = ((uint64_t)hash[2 * i] << 32) | hash[2 * i + 1];
+    std::stable_sort(v.begin(), v.end(),
+                     [](uint64_t a, uint64_t b) { return (a >> 32) < (b >> 32); });
+    for (uint32_t i = 0; i < n; ++i) {
+        hash[2 * i] = (uint32_t)(v[i] >> 32);
+        hash[2 * i + 1] = (uint32_t)v[i];
+    }
+}
+
+// Parallel LSD radix sort (stable), 8-bit digits over the significant key bits.
+void orc_sort_radix(uint32_t* hash, uint32_t n, uint32_t num_cells) {
+    set_threads();
+    std::vector<uint32_t> tmp;
+    radix_sort_pairs(hash, n, num_cells, tmp);
+}
+
+// reorderDataFloat4
+void orc_reorder(const uint32_t* hash, uint32_t n, const float* in, float* out) {
+    set_threads();
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        uint32_t id = hash[2 * i + 1];
+        std::memcpy(out + 4 * i, in + 4 * (size_t)id, 16);
+    }
+}
+
+// genericFindCellStartD with explicit empty-cell value and cell end.
+void orc_cell_ranges(const uint32_t* hash, uint32_t n, uint32_t num_cells, uint32_t* start,
+                     uint32_t* end) {
+    set_threads();
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < (int64_t)num_cells; ++c) {
+        start[c] = BRT_EMPTY_CELL;
+        end[c] = BRT_EMPTY_CELL;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        uint32_t k = hash[2 * i];
+        if (i == 0 || hash[2 * (i - 1)] != k) start[k] = (uint32_t)i;
+        if (i == (int64_t)n - 1 || hash[2 * (i + 1)] != k) end[k] = (uint32_t)(i + 1);
+    }
+}
+
+// genericCalcNeighgborhoodD: 27 cells, x outermost, z innermost, each cell in
+// sorted order, truncated at max_bodies_per_cell, self excluded,
+// "sqrtf(d2) > r -> reject", capped at K (or KNN replacement).
+void orc_neighbors(const brt_grid_params* g, const brt_prox_params* p, const float* spos,
+                   const uint32_t* hash, const uint32_t* start, const uint32_t* end, uint32_t n,
+                   uint32_t* lists) {
+    set_threads();
+    const uint32_t K = p->max_neighbors;
+    const uint32_t W = listStride(K);
+    const float r = p->search_radius;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const float* pi = spos + 4 * (size_t)i;
+        I3 gp = calcGridPos(pi, *g);
+        uint32_t cnt = 0;
+        uint32_t nl[BRT_MAX_NEIGHBORS + 4];
+        float dl[BRT_MAX_NEIGHBORS + 4];
+        float maxDist = 0.0f;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    I3 c = {gp.x + dx, gp.y + dy, gp.z + dz};
+                    if (!inGrid(c, *g)) continue;
+                    uint32_t key = calcGridHash(c, *g);
+                    uint32_t s = start[key];
+                    if (s == BRT_EMPTY_CELL) continue;
+                    for (uint32_t t = 0; t < g->max_bodies_per_cell; ++t) {
+                        uint32_t j = s + t;
+                        if (j >= n || hash[2 * (size_t)j] != key) break;
+                        if (j == i) continue;
+                        const float* pj = spos + 4 * (size_t)j;
+                        float rx = pj[0] - pi[0], ry = pj[1] - pi[1], rz = pj[2] - pi[2];
+                        float xx = rx * rx, yy = ry * ry, zz = rz * rz;
+                        float dist = sqrtf((xx + yy) + zz);
+                        if (dist > r) continue;
+                        if (cnt < K) {
+                            nl[cnt] = j;
+                            if (p->use_knn) {
+                                dl[cnt] = dist;
+                                maxDist = std::max(maxDist, dist);
+                            }
+                            ++cnt;
+                        } else if (p->use_knn && dist < maxDist) {
+                            maxDist = dl[0];
+                            uint32_t mi = 0;
+                            for (uint32_t q = 1; q < cnt; ++q)
+                                if (maxDist < dl[q]) {
+                                    maxDist = dl[q];
+                                    mi = q;
+                                }
+                            nl[mi] = j;
+                            dl[mi] = dist;
+                        }
+                    }
+                }
+        uint32_t* L = lists + (size_t)i * W;
+        L[0] = cnt;
+        for (uint32_t q = 0; q < cnt; ++q) L[1 + q] = nl[q];
+        for (uint32_t q = cnt + 1; q < W; ++q) L[q] = BRT_LIST_PAD;
+    }
+}
+
+// Independent O(N^2) formulation for tiny N: enumerate every pair, place each
+// accepted candidate at (canonical cell index, sorted index), sort, keep K.
+// Non-KNN only.
+void orc_neighbors_brute(const brt_grid_params* g, const brt_prox_params* p, const float* spos,
+                         const uint32_t* hash, uint32_t n, uint32_t* lists) {
+    const uint32_t K = p->max_neighbors;
+    const uint32_t W = listStride(K);
+    std::vector<uint32_t> first(n);
+    for (uint32_t j = 0; j < n; ++j) {
+        uint32_t f = j;
+        while (f > 0 && hash[2 * (f - 1)] == hash[2 * j]) --f;
+        first[j] = f;
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> cand;
+    for (uint32_t i = 0; i < n; ++i) {
+        I3 gp = calcGridPos(spos + 4 * (size_t)i, *g);
+        cand.clear();
+        for (uint32_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            int order = -1;
+            for (int o = 0; o < 27 && order < 0; ++o) {
+                I3 c = {gp.x + o / 9 - 1, gp.y + (o / 3) % 3 - 1, gp.z + o % 3 - 1};
+                if (inGrid(c, *g) && calcGridHash(c, *g) == hash[2 * j]) order = o;
+            }
+            if (order < 0) continue;
+            if (j - first[j] >= g->max_bodies_per_cell) continue;
+            // single-precision test exactly as the reference evaluates it
+            float rx = spos[4 * j] - spos[4 * i], ry = spos[4 * j + 1] - spos[4 * i + 1],
+                  rz = spos[4 * j + 2] - spos[4 * i + 2];
+            float xx = rx * rx, yy = ry * ry, zz = rz * rz;
+            if (sqrtf((xx + yy) + zz) > p->search_radius) continue;
+            cand.push_back({(uint32_t)order, j});
+        }
+        std::sort(cand.begin(), cand.end());
+        uint32_t cnt = (uint32_t)std::min<size_t>(cand.size(), K);
+        uint32_t* L = lists + (size_t)i * W;
+        L[0] = cnt;
+        for (uint32_t q = 0; q < cnt; ++q) L[1 + q] = cand[q].second;
+        for (uint32_t q = cnt + 1; q < W; ++q) L[q] = BRT_LIST_PAD;
+    }
+}
+
+void orc_steer_f32(const brt_grid_params* g, const brt_prox_params* p, const brt_steer_params* s,
+                   float dt, uint32_t step, uint32_t n, const float* spos, const float* sfwd,
+                   const float* ssa, const uint32_t* hash, const uint32_t* lists, float* wander,
+                   float* pos_out, float* fwd_out, float* sa_out, float* force_out) {
+    set_threads();
+    SteerArrays<float> A{spos, sfwd, ssa, hash, lists, listStride(p->max_neighbors),
+                         wander, pos_out, fwd_out, sa_out, force_out};
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) steer_agent<float>((uint32_t)i, *g, *s, dt, step, A);
+}
+
+void orc_steer_f64(const brt_grid_params* g, const brt_prox_params* p, const brt_steer_params* s,
+                   float dt, uint32_t step, uint32_t n, const double* spos, const double* sfwd,
+                   const double* ssa, const uint32_t* hash, const uint32_t* lists, double* wander,
+                   double* pos_out, double* fwd_out, double* sa_out, double* force_out) {
+    set_threads();
+    SteerArrays<double> A{spos, sfwd, ssa, hash, lists, listStride(p->max_neighbors),
+                          wander, pos_out, fwd_out, sa_out, force_out};
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) steer_agent<double>((uint32_t)i, *g, *s, (double)dt, step, A);
+}
+
+void orc_feature_update_f32(const brt_prox_params* p, float rate, uint32_t n, const uint32_t* hash,
+                            const uint32_t* lists, const float* fin, float* fout) {
+    set_threads();
+    const uint32_t W = listStride(p->max_neighbors);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) feature_agent<float>((uint32_t)i, rate, hash, lists, W, fin, fout);
+}
+
+void orc_feature_update_f64(const brt_prox_params* p, float rate, uint32_t n, const uint32_t* hash,
+                            const uint32_t* lists, const double* fin, double* fout) {
+    set_threads();
+    const uint32_t W = listStride(p->max_neighbors);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+        feature_agent<double>((uint32_t)i, (double)rate, hash, lists, W, fin, fout);
+}
+
+// Whole FP32 step on host buffers laid out as brt_step_buffers (host pointers).
+// Phase wall times (seconds) are added into phase_s[0..4] when non-null.
+void orc_step_f32(const brt_grid_params* g, const brt_prox_params* p, const brt_steer_params* s,
+                  float feature_rate, float dt, uint32_t step, const brt_step_buffers* b,
+                  double* phase_s) {
+    set_threads();
+    const uint32_t n = b->n;
+    auto now = []() {
+#ifdef _OPENMP
+        return omp_get_wtime();
+#else
+        return 0.0;
+#endif
+    };
+    double t0 = now();
+    orc_calc_hash(g, b->pos, n, b->hash);
+    orc_sort_radix(b->hash, n, g->num_cells);
+    double t1 = now();
+    orc_reorder(b->hash, n, b->pos, b->sorted_pos);
+    orc_reorder(b->hash, n, b->forward, b->sorted_forward);
+    orc_reorder(b->hash, n, b->smooth_accel, b->sorted_smooth_accel);
+    orc_cell_ranges(b->hash, n, g->num_cells, b->cell_start, b->cell_end);
+    double t2 = now();
+    orc_neighbors(g, p, b->sorted_pos, b->hash, b->cell_start, b->cell_end, n, b->lists);
+    double t3 = now();
+    orc_steer_f32(g, p, s, dt, step, n, b->sorted_pos, b->sorted_forward, b->sorted_smooth_accel,
+                  b->hash, b->lists, b->wander, b->pos, b->forward, b->smooth_accel, nullptr);
+    double t4 = now();
+    if (b->features && b->features_out && feature_rate != 0.0f)
+        orc_feature_update_f32(p, feature_rate, n, b->hash, b->lists, b->features, b->features_out);
+    double t5 = now();
+    if (phase_s) {
+        phase_s[0] += t1 - t0;
+        phase_s[1] += t2 - t1;
+        phase_s[2] += t3 - t2;
+        phase_s[3] += t4 - t3;
+        phase_s[4] += t5 - t4;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+"""GPU parity of EnvGrid3D (hash, onesweep sort, reorder, cell ranges) through the
+C ABI against the CPU oracle: bit-exact."""
+import numpy as np
+import pytest
+import torch
+
+from behavert_amd import workloads as W
+from behavert_amd.types import GridParams
+
+from ._parity import dev, empty_f32, empty_u32, f32, first_mismatch, u32
+
+pytestmark = pytest.mark.gpu
+
+
+def pos4(xyz):
+    xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
+    p = np.ones((len(xyz), 4), np.float32)
+    p[:, :3] = xyz
+    return p
+
+
+def test_hash_bit_exact_including_edges(ctx, orc):
+    g = W.cube_grid(100.0)
+    rng = np.random.default_rng(3)
+    xyz = rng.uniform(-10, 110, (10000, 3)).astype(np.float32)
+    xyz[:25] = np.float32(4.0) * np.arange(25, dtype=np.float32)[:, None]
+    xyz[25] = [np.nan, 5, 5]
+    xyz[26] = [3e38, -3e38, np.inf]
+    p = pos4(xyz)
+    h = empty_u32(len(p), 2)
+    ctx.calc_hash(g, dev(p), h)
+    assert first_mismatch(u32(h), orc.calc_hash(g, p)) is None
+
+
+def test_hash_non_cubic_grid(ctx, orc):
+    g = GridParams.direct(origin=(-10.0, 5.0, 2.5), cell=(2.0, 3.0, 0.5), size=(7, 11, 13))
+    rng = np.random.default_rng(4)
+    p = pos4(rng.uniform(-20, 40, (5000, 3)))
+    h = empty_u32(len(p), 2)
+    ctx.calc_hash(g, dev(p), h)
+    assert first_mismatch(u32(h), orc.calc_hash(g, p)) is None
+
+
+SORT_CASES = [(1, 10), (2, 2), (1000, 7), (4095, 15625), (4096, 15625), (4097, 15625), (65543, 1_000_000),
+              (300_000, 256), (300_000, 257), (1 << 20, 1_000_000), (1 << 20, 4_096_000), (200_003, 2 ** 31 - 1),
+              (100_000, 1)]
+
+
+@pytest.mark.parametrize("n,cells", SORT_CASES)
+def test_sort_bit_exact(ctx, n, cells):
+    rng = np.random.default_rng(n + cells)
+    k = rng.integers(0, cells, n, dtype=np.uint64).astype(np.uint32)
+    h = np.stack([k, np.arange(n, dtype=np.uint32)], 1)
+    d = dev(h.view(np.int32))
+    ctx.sort(d, cells)
+    expect = h[np.lexsort((h[:, 1], h[:, 0]))]
+    assert first_mismatch(u32(d), expect) is None
+
+
+def test_sort_skewed_single_hot_key(ctx):
+    n = 500_000
+    rng = np.random.default_rng(9)
+    k = np.where(rng.random(n) < 0.9, 123456, rng.integers(0, 1_000_000, n)).astype(np.uint32)
+    h = np.stack([k, np.arange(n, dtype=np.uint32)], 1)
+    d = dev(h.view(np.int32))
+    ctx.sort(d, 1_000_000)
+    assert first_mismatch(u32(d), h[np.lexsort((h[:, 1], h[:, 0]))]) is None
+
+
+@pytest.mark.parametrize("make", [W.plain_4096, W.blobs_1m])
+def test_hash_sort_reorder_ranges_bit_exact(ctx, orc, make):
+    w = make()
+    st = w.state
+    n = w.n
+    hd = empty_u32(n, 2)
+    pos = dev(st["pos"])
+    fwd = dev(st["forward"])
+    sa = dev(st["smooth_accel"])
+    ctx.hash_sort(w.grid, pos, hd)
+    h = orc.sort_radix(orc.calc_hash(w.grid, st["pos"]), w.grid.num_cells)
+    assert first_mismatch(u32(hd), h) is None
+    outs = [empty_f32(n, 4) for _ in range(3)]
+    cs = empty_u32(w.grid.num_cells)
+    ce = empty_u32(w.grid.num_cells)
+    ctx.reorder_ranges(hd, [pos, fwd, sa], outs, w.grid.num_cells, cs, ce)
+    s, e = orc.cell_ranges(h, w.grid.num_cells)
+    assert first_mismatch(u32(cs), s) is None
+    assert first_mismatch(u32(ce), e) is None
+    for o, key in zip(outs, ("pos", "forward", "smooth_accel")):
+        assert first_mismatch(f32(o).view(np.uint32), orc.reorder(h, st[key]).view(np.uint32)) is None
+    # the separate entry points agree with the fused one
+    outs2 = [empty_f32(n, 4) for _ in range(2)]
+    ctx.reorder(hd, [pos, fwd], outs2)
+    assert np.array_equal(f32(outs2[0]), f32(outs[0])) and np.array_equal(f32(outs2[1]), f32(outs[1]))
+    cs2 = empty_u32(w.grid.num_cells)
+    ce2 = empty_u32(w.grid.num_cells)
+    ctx.cell_ranges(hd, w.grid.num_cells, cs2, ce2)
+    assert np.array_equal(u32(cs2), s) and np.array_equal(u32(ce2), e)
+
+
+def test_sort_repeated_calls_reuse_scratch(ctx):
+    """Back-to-back sorts of different sizes on one context (scratch reuse and
+    re-clearing of look-back state)."""
+    rng = np.random.default_rng(21)
+    for n in (100_000, 5000, 300_000, 1):
+        k = rng.integers(0, 40_000, n).astype(np.uint32)
+        h = np.stack([k, np.arange(n, dtype=np.uint32)], 1)
+        d = dev(h.view(np.int32))
+        ctx.sort(d, 40_000)
+        assert first_mismatch(u32(d), h[np.lexsort((h[:, 1], h[:, 0]))]) is None
+    torch.cuda.synchronize()

@@ -1,0 +1,219 @@
+"""GPU parity of the fused OpenSteerWrapper steering + Body integration, the
+BehaviorClustering feature update and the whole step (brt_step) through the C
+ABI: bit-exact against the FP32 oracle, and within the stated tolerances of the
+FP64 evaluation (per agent, norm-wise, floored at each quantity's scale:
+force 1.0, smooth_accel 0.4 = rate * unit force, forward 1.0, pos 1.0)."""
+import numpy as np
+import pytest
+
+from behavert_amd import workloads as W
+from behavert_amd.engine import Flock
+from behavert_amd.types import GridParams, ProxParams, SteerParams
+
+from ._parity import dev, empty_f32, f32, first_mismatch, floored_rel, oracle_grid_outputs, u32
+
+pytestmark = pytest.mark.gpu
+
+FLOORS = {"force": 1.0, "smooth_accel": 0.4, "forward": 1.0, "pos": 1.0}
+
+
+def gpu_steer(ctx, w, o, steer=None, step=0, wander=None):
+    n = w.n
+    outs = [empty_f32(n, 4) for _ in range(4)]
+    wd = None if wander is None else dev(wander)
+    ctx.steer_integrate(w.grid, w.prox, steer or w.steer, w.dt, step, dev(o["sorted_pos"]),
+                        dev(o["sorted_forward"]), dev(o["sorted_smooth_accel"]), dev(o["hash"].view(np.int32)),
+                        dev(o["lists"].view(np.int32)), outs[0], outs[1], outs[2], wander=wd, force_out=outs[3])
+    res = {"pos": f32(outs[0]), "forward": f32(outs[1]), "smooth_accel": f32(outs[2]), "force": f32(outs[3])}
+    if wd is not None:
+        res["wander"] = f32(wd)
+    return res
+
+
+@pytest.mark.parametrize("make", [W.plain_4096, W.blobs_1m])
+def test_one_step_bit_exact_vs_fp32_and_within_1e5_of_fp64(ctx, orc, make, record_property):
+    w = make()
+    o = oracle_grid_outputs(orc, w.grid, w.prox, w.state)
+    got = gpu_steer(ctx, w, o)
+    ref = orc.steer_f32(w.grid, w.prox, w.steer, w.dt, 0, o["sorted_pos"], o["sorted_forward"],
+                        o["sorted_smooth_accel"], o["hash"], o["lists"])
+    for k in ("force", "pos", "forward", "smooth_accel"):
+        assert first_mismatch(got[k].view(np.uint32), ref[k].view(np.uint32)) is None, k
+    r64 = orc.steer_f64(w.grid, w.prox, w.steer, w.dt, 0, o["sorted_pos"].astype(np.float64),
+                        o["sorted_forward"].astype(np.float64), o["sorted_smooth_accel"].astype(np.float64),
+                        o["hash"], o["lists"])
+    for k, floor in FLOORS.items():
+        err = floored_rel(got[k][:, :3], r64[k][:, :3], floor)
+        record_property(f"{w.name}_{k}_max_err_vs_fp64", float(err.max()))
+        print(f"{w.name} {k}: max floored rel err vs FP64 = {err.max():.3e}, p99 = {np.quantile(err, 0.99):.3e}")
+        assert err.max() < 1e-5, k
+
+
+def test_behaviour_subsets_and_options_bit_exact(ctx, orc):
+    rng = np.random.default_rng(31)
+    grid = GridParams.direct((0.0,) * 3, (4.0,) * 3, (10,) * 3)
+    st = W.make_state(rng.uniform(0, 40, (30000, 3)).astype(np.float32), *W.directions_and_speeds(rng, 30000))
+    # some slow agents exercise the cone-limit branch
+    st["forward"][:3000, 3] = rng.uniform(0.0, 0.2, 3000).astype(np.float32)
+    st["smooth_accel"][:, :3] = rng.uniform(-0.1, 0.1, (30000, 3)).astype(np.float32)
+    prox = ProxParams.make(16, 4.0)
+    w = W.Workload("mix", 30000, 40.0, grid, prox, W.plain_steer(), 0.05, 1, st)
+    o = oracle_grid_outputs(orc, grid, prox, st)
+    variants = [
+        SteerParams.make(weights=(1.0, 0.0, 0.0)),
+        SteerParams.make(weights=(0.0, 0.8, 0.0), cos_max=0.3),
+        SteerParams.make(weights=(0.0, 0.0, 0.6), radius=2.5, min_radius=0.5),
+        SteerParams.make(weights=(1.0, 0.8, 0.6), seek_world_center=True),
+        SteerParams.make(weights=(1.0, 0.8, 0.6), use_2d=True, mass=2.0, max_force=0.2),
+        SteerParams.make(weights=(1.0, 0.8, 0.6), targets=W.moving_targets(1.5, 20.0), target_weight=0.5),
+    ]
+    for sp in variants:
+        got = gpu_steer(ctx, w, o, steer=sp)
+        ref = orc.steer_f32(grid, prox, sp, w.dt, 0, o["sorted_pos"], o["sorted_forward"],
+                            o["sorted_smooth_accel"], o["hash"], o["lists"])
+        for k in ("force", "pos", "forward", "smooth_accel"):
+            assert first_mismatch(got[k].view(np.uint32), ref[k].view(np.uint32)) is None, k
+
+
+def test_wander_rng_stream_and_state_bit_exact(ctx, orc):
+    w = W.plain_4096()
+    sp = W.plain_steer()
+    sp.wander_weight = 0.3
+    sp.wander_seed = 777
+    wander = np.random.default_rng(1).uniform(-1, 1, (w.n, 2)).astype(np.float32)
+    o = oracle_grid_outputs(orc, w.grid, w.prox, w.state)
+    for step in (0, 1, 123456):
+        got = gpu_steer(ctx, w, o, steer=sp, step=step, wander=wander)
+        ref = orc.steer_f32(w.grid, w.prox, sp, w.dt, step, o["sorted_pos"], o["sorted_forward"],
+                            o["sorted_smooth_accel"], o["hash"], o["lists"], wander=wander)
+        assert first_mismatch(got["wander"].view(np.uint32), ref["wander"].view(np.uint32)) is None
+        for k in ("force", "pos", "forward", "smooth_accel"):
+            assert first_mismatch(got[k].view(np.uint32), ref[k].view(np.uint32)) is None, k
+
+
+def test_feature_update_bit_exact_and_vs_fp64(ctx, orc):
+    w = W.plain_4096()
+    o = oracle_grid_outputs(orc, w.grid, w.prox, w.state)
+    feats = np.random.default_rng(2).random((w.n, 4)).astype(np.float32)
+    fin = dev(feats)
+    fout = empty_f32(w.n, 4)
+    ctx.feature_update(w.prox, 0.1, dev(o["hash"].view(np.int32)), dev(o["lists"].view(np.int32)), fin, fout)
+    got = f32(fout)
+    ref = orc.feature_update_f32(w.prox, 0.1, o["hash"], o["lists"], feats)
+    assert first_mismatch(got.view(np.uint32), ref.view(np.uint32)) is None
+    r64 = orc.feature_update_f64(w.prox, 0.1, o["hash"], o["lists"], feats.astype(np.float64))
+    assert floored_rel(got, r64, 1.0).max() < 1e-6
+
+
+def run_gpu_steps(ctx, w, steps, fused=True):
+    fl = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander)
+    for i in range(steps):
+        if w.moving_targets:
+            fl.steer.set_targets(w.targets_at(i))
+        if fused:
+            fl.step(w.dt, i, w.feature_rate)
+        else:
+            fl.reorder_sim_data()
+            fl.compute_neighborhoods()
+            fl.apply_steering(w.dt, i)
+            if w.features is not None and w.feature_rate:
+                fl.update_features(w.feature_rate)
+    return fl
+
+
+def run_oracle_steps(orc, w, steps, keep=False):
+    hf = orc.HostFlock(w.grid, w.prox, w.state, features=w.features, wander=w.wander)
+    hist = []
+    steer = SteerParams()
+    import ctypes
+    ctypes.memmove(ctypes.byref(steer), ctypes.byref(w.steer), ctypes.sizeof(SteerParams))
+    for i in range(steps):
+        if w.moving_targets:
+            steer.set_targets(w.targets_at(i))
+        hf.step(steer, w.dt, i, w.feature_rate)
+        if keep:
+            hist.append((hf.hash.copy(), hf.lists.copy()))
+    return hf, hist
+
+
+def test_step_fused_equals_decomposed_and_oracle_over_10_steps(ctx, orc):
+    w = W.plain_4096()
+    a = run_gpu_steps(ctx, w, 10, fused=True)
+    b = run_gpu_steps(ctx, w, 10, fused=False)
+    sa, sb = a.state(), b.state()
+    for k in ("pos", "forward", "smooth_accel"):
+        assert np.array_equal(sa[k].view(np.uint32), sb[k].view(np.uint32)), k
+    hf, hist = run_oracle_steps(orc, w, 10, keep=True)
+    ref = hf.state()
+    for k in ("pos", "forward", "smooth_accel"):
+        assert first_mismatch(sa[k].view(np.uint32), ref[k].view(np.uint32)) is None, k
+    go = a.grid_outputs()
+    assert first_mismatch(go["hash"], hf.hash) is None
+    assert first_mismatch(go["lists"], hf.lists) is None
+
+
+def test_ten_step_trajectory_within_1e3_of_fp64(ctx, orc, record_property):
+    """10 steps on the GPU vs an FP64 trajectory that uses the same per-step sort
+    and neighbour lists (those of the FP32 run, identical on GPU and oracle)."""
+    w = W.plain_4096()
+    fl = run_gpu_steps(ctx, w, 10)
+    got = fl.state()
+    _, hist = run_oracle_steps(orc, w, 10, keep=True)
+    s64 = {k: w.state[k].astype(np.float64) for k in ("pos", "forward", "smooth_accel")}
+    for i, (h, L) in enumerate(hist):
+        out = orc.step_f64_fixed(w.grid, w.prox, w.steer, w.dt, i, s64, h, L)
+        s64 = {k: out[k] for k in ("pos", "forward", "smooth_accel")}
+    for k, floor in FLOORS.items():
+        if k == "force":
+            continue
+        err = floored_rel(got[k][:, :3], s64[k][:, :3], floor)
+        record_property(f"traj10_{k}_max_err", float(err.max()))
+        print(f"10-step {k}: max floored rel err vs FP64 = {err.max():.3e}")
+        assert err.max() < 1e-3, k
+
+
+def test_full_stack_small_bit_exact(ctx, orc):
+    """Features + wander + moving-target seek at 65,536 agents (the 4M workload's
+    composition at a size the oracle runs in seconds), 3 steps."""
+    w = W.full_stack_4m(n=1 << 16)
+    w.side = 640.0 * (1 / 64) ** (1 / 3)
+    st = W.uniform_flock(w.n, w.side, 0)
+    w.state = st
+    w.grid = W.cube_grid(w.side)
+    fl = run_gpu_steps(ctx, w, 3)
+    hf, _ = run_oracle_steps(orc, w, 3)
+    got, ref = fl.state(), hf.state()
+    for k in ("pos", "forward", "smooth_accel", "features", "wander"):
+        assert first_mismatch(got[k].view(np.uint32), ref[k].view(np.uint32)) is None, k
+
+
+def test_phase_timing_reports_all_phases(ctx):
+    w = W.plain_4096()
+    ctx.phase_timing(True)
+    run_gpu_steps(ctx, w, 3)
+    ms, steps = ctx.phase_times()
+    ctx.phase_timing(False)
+    assert steps == 3
+    assert all(v >= 0 for v in ms.values()) and ms["neighbors"] > 0
+
+
+def test_error_paths_return_codes(ctx):
+    from behavert_amd import BrtError
+    w = W.plain_4096()
+    bad = W.cube_grid(100.0)
+    bad.num_cells = 7
+    with pytest.raises(BrtError):
+        ctx.calc_hash(bad, dev(w.state["pos"]), dev(np.zeros((w.n, 2), np.int32)))
+    with pytest.raises(BrtError):
+        ctx.neighbors(w.grid, ProxParams.make(0, 4.0), dev(w.state["pos"]), dev(np.zeros((w.n, 2), np.int32)),
+                      dev(np.zeros(15625, np.int32)), dev(np.zeros(15625, np.int32)),
+                      dev(np.zeros((w.n, 4), np.int32)))
+    with pytest.raises(BrtError):
+        ctx.neighbors(w.grid, ProxParams.make(300, 4.0), dev(w.state["pos"]), dev(np.zeros((w.n, 2), np.int32)),
+                      dev(np.zeros(15625, np.int32)), dev(np.zeros(15625, np.int32)),
+                      dev(np.zeros((w.n, 4), np.int32)))
+    sp = W.plain_steer()
+    sp.wander_weight = 1.0
+    fl = Flock(ctx, w.grid, w.prox, sp, w.state)
+    with pytest.raises(BrtError):
+        fl.step(w.dt, 0)
