@@ -26,6 +26,8 @@ SIGNATURES = {
     "brt_context_destroy": (_I, [_P]),
     "brt_context_last_error": (C.c_char_p, [_P]),
     "brt_version": (C.c_char_p, []),
+    "brt_context_set_option": (_I, [_P, _I, _I]),
+    "brt_context_get_option": (_I, [_P, _I, C.POINTER(_I)]),
     "brt_grid_setup": (_I, [C.POINTER(_F), C.POINTER(_F), C.POINTER(_F), _U32, _GP]),
     "brt_list_stride": (_U32, [_U32]),
     "brt_radius_threshold": (_F, [_F]),

@@ -40,6 +40,24 @@ def hipcc() -> str:
     return path
 
 
+EXAMPLE_SRC = os.path.join(HERE, "examples", "flocking_demo.cpp")
+EXAMPLE_BIN = os.path.join(HERE, "examples", "flocking_demo")
+
+
+def build_example(verbose: bool = False) -> str:
+    """The plug-in layer's BehaveRT_Flocking-style driver, linked to the library."""
+    cmd = [hipcc(), "-std=c++17", "-O2", "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-Wall", "-Wno-unused-command-line-argument", "-o", EXAMPLE_BIN, EXAMPLE_SRC,
+           "-L", HERE, "-lbehavert_amd", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout + proc.stderr)
+        raise RuntimeError(f"building the plug-in example failed with exit code {proc.returncode}")
+    return EXAMPLE_BIN
+
+
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
@@ -53,6 +71,7 @@ def build(verbose: bool = False) -> str:
     if verbose and proc.stderr.strip():
         sys.stderr.write(proc.stderr)
     os.replace(tmp, LIB)
+    build_example(verbose)
     return LIB
 
 

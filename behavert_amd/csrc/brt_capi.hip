@@ -29,6 +29,9 @@ struct brt_context {
     size_t events_used = 0;
     double phase_ms[BRT_NUM_PHASES] = {0, 0, 0, 0, 0};
     uint32_t phase_steps = 0;
+    int fused_step = 1;
+    uint32_t uniform_max_cells = 1;
+    uint32_t xcd_remap = 1;
 };
 
 namespace {
@@ -194,6 +197,41 @@ int brt_context_destroy(brt_context* ctx) {
 
 const char* brt_context_last_error(const brt_context* ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
 
+int brt_context_set_option(brt_context* ctx, int option, int value) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    switch (option) {
+        case BRT_OPT_FUSED_STEP:
+            ctx->fused_step = value != 0;
+            return BRT_OK;
+        case BRT_OPT_UNIFORM_MAX_CELLS:
+            BRT_REQUIRE(value >= 0 && value <= 64, "uniform_max_cells must be in 0..64");
+            ctx->uniform_max_cells = (uint32_t)value;
+            return BRT_OK;
+        case BRT_OPT_XCD_REMAP:
+            ctx->xcd_remap = value != 0;
+            return BRT_OK;
+        default:
+            return set_err(ctx, BRT_ERR_INVALID_ARGUMENT, "unknown option %d", option);
+    }
+}
+
+int brt_context_get_option(const brt_context* ctx, int option, int* value) {
+    if (!ctx || !value) return BRT_ERR_INVALID_ARGUMENT;
+    switch (option) {
+        case BRT_OPT_FUSED_STEP:
+            *value = ctx->fused_step;
+            return BRT_OK;
+        case BRT_OPT_UNIFORM_MAX_CELLS:
+            *value = (int)ctx->uniform_max_cells;
+            return BRT_OK;
+        case BRT_OPT_XCD_REMAP:
+            *value = (int)ctx->xcd_remap;
+            return BRT_OK;
+        default:
+            return BRT_ERR_INVALID_ARGUMENT;
+    }
+}
+
 int brt_grid_setup(const float world_center[3], const float world_radius[3], const float cell_size[3],
                    uint32_t max_bodies_per_cell, brt_grid_params* out) {
     if (!world_center || !world_radius || !cell_size || !out) return BRT_ERR_INVALID_ARGUMENT;
@@ -353,7 +391,8 @@ int brt_prox_neighbors(brt_context* ctx, const brt_grid_params* grid, const brt_
     if ((rc = use_device(ctx))) return rc;
     BRT_HIP(brt::launch_neighbors(*grid, *prox, brt_radius_threshold(prox->search_radius),
                                   reinterpret_cast<const float4*>(sorted_pos), reinterpret_cast<const uint2*>(hash),
-                                  cell_start, cell_end, n, lists, (hipStream_t)stream),
+                                  cell_start, cell_end, n, lists, ctx->uniform_max_cells, ctx->xcd_remap,
+                                  (hipStream_t)stream),
             "k_neighbors");
     return BRT_OK;
 }
@@ -386,7 +425,7 @@ int brt_osw_steer_integrate(brt_context* ctx, const brt_grid_params* grid, const
                               reinterpret_cast<const uint2*>(hash), lists, reinterpret_cast<float2*>(wander),
                               reinterpret_cast<float4*>(pos_out), reinterpret_cast<float4*>(forward_out),
                               reinterpret_cast<float4*>(smooth_accel_out), reinterpret_cast<float4*>(force_out),
-                              (hipStream_t)stream),
+                              ctx->xcd_remap, (hipStream_t)stream),
             "k_steer");
     return BRT_OK;
 }
@@ -463,23 +502,40 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
                                        b->cell_end, s),
             "k_reorder_ranges");
     if (ev) BRT_HIP(hipEventRecord((*ev)[2], s), "hipEventRecord");
-    // phase 2: neighbour lists
-    BRT_HIP(brt::launch_neighbors(*grid, *prox, brt_radius_threshold(prox->search_radius),
+    const float T = brt_radius_threshold(prox->search_radius);
+    if (ctx->fused_step && !prox->use_knn) {
+        // phase 2: neighbour lists + steering + integration in one pass
+        BRT_HIP(brt::launch_neighbors_steer(*grid, *prox, T, *steer, dt, step,
+                                            reinterpret_cast<const float4*>(b->sorted_pos),
+                                            reinterpret_cast<const float4*>(b->sorted_forward),
+                                            reinterpret_cast<const float4*>(b->sorted_smooth_accel),
+                                            reinterpret_cast<const uint2*>(b->hash), b->cell_start, b->cell_end, n,
+                                            b->lists, reinterpret_cast<float2*>(b->wander),
+                                            reinterpret_cast<float4*>(b->pos), reinterpret_cast<float4*>(b->forward),
+                                            reinterpret_cast<float4*>(b->smooth_accel), ctx->uniform_max_cells,
+                                            ctx->xcd_remap, s),
+                "k_prox<fused>");
+        if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
+        if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
+    } else {
+        // phase 2: neighbour lists
+        BRT_HIP(brt::launch_neighbors(*grid, *prox, T, reinterpret_cast<const float4*>(b->sorted_pos),
+                                      reinterpret_cast<const uint2*>(b->hash), b->cell_start, b->cell_end, n,
+                                      b->lists, ctx->uniform_max_cells, ctx->xcd_remap, s),
+                "k_neighbors");
+        if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
+        // phase 3: fused steering + integration, scattered back to agent-ID order
+        BRT_HIP(brt::launch_steer(*grid, prox->max_neighbors, *steer, dt, step, n,
                                   reinterpret_cast<const float4*>(b->sorted_pos),
-                                  reinterpret_cast<const uint2*>(b->hash), b->cell_start, b->cell_end, n, b->lists, s),
-            "k_neighbors");
-    if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
-    // phase 3: fused steering + integration, scattered back to agent-ID order
-    BRT_HIP(brt::launch_steer(*grid, prox->max_neighbors, *steer, dt, step, n,
-                              reinterpret_cast<const float4*>(b->sorted_pos),
-                              reinterpret_cast<const float4*>(b->sorted_forward),
-                              reinterpret_cast<const float4*>(b->sorted_smooth_accel),
-                              reinterpret_cast<const uint2*>(b->hash), b->lists,
-                              reinterpret_cast<float2*>(b->wander), reinterpret_cast<float4*>(b->pos),
-                              reinterpret_cast<float4*>(b->forward), reinterpret_cast<float4*>(b->smooth_accel),
-                              nullptr, s),
-            "k_steer");
-    if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
+                                  reinterpret_cast<const float4*>(b->sorted_forward),
+                                  reinterpret_cast<const float4*>(b->sorted_smooth_accel),
+                                  reinterpret_cast<const uint2*>(b->hash), b->lists,
+                                  reinterpret_cast<float2*>(b->wander), reinterpret_cast<float4*>(b->pos),
+                                  reinterpret_cast<float4*>(b->forward), reinterpret_cast<float4*>(b->smooth_accel),
+                                  nullptr, ctx->xcd_remap, s),
+                "k_steer");
+        if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
+    }
     // phase 4: BehaviorClustering feature update
     if (do_features)
         BRT_HIP(brt::launch_features(prox->max_neighbors, feature_rate, n, reinterpret_cast<const uint2*>(b->hash),

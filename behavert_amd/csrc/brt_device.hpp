@@ -6,6 +6,9 @@
 //   truncateLength, adjustRawSteeringForce, vecLimitDeviationAngleUtility,
 //   blendIntoAccumulator3, interpolate Core/common_resources.cu:140-271
 //   inBoidNeighborhood, xxxsteerForSeek PlugIns/OpenSteerWrapper/OpenSteerWrapper_resources.cu:67-92,294-306
+// Fixed choices where cutil_math (not in the reference tree) leaves the last bit
+// open: normalize(v) = v * (1/sqrtf(dot)), v / s = v * (1/s), powf(x,20) by
+// repeated squaring.
 // Every TU that includes this is compiled with -ffp-contract=off, so each
 // a*b+c below is a rounded multiply followed by a rounded add, and sqrtf and
 // '/' are the correctly rounded IEEE operations (HIP's default for FP32).
@@ -26,7 +29,11 @@ __device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}
 __device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
 __device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ V3 operator/(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+// cutil_math float3 / float: one correctly rounded reciprocal, three multiplies
+__device__ __forceinline__ V3 operator/(V3 a, float s) {
+    const float inv = 1.0f / s;
+    return a * inv;
+}
 
 // cutil dot order: (a.x*b.x + a.y*b.y) + a.z*b.z
 __device__ __forceinline__ float dot(V3 a, V3 b) {
@@ -144,6 +151,21 @@ __device__ __forceinline__ uint32_t calcGridHash(I3 gp, const brt_grid_params& g
 __device__ __forceinline__ bool inGrid(int x, int y, int z, const brt_grid_params& g) {
     return x >= 0 && x <= (int)g.grid_size[0] - 1 && y >= 0 && y <= (int)g.grid_size[1] - 1 && z >= 0 &&
            z <= (int)g.grid_size[2] - 1;
+}
+
+// ------------------------------------------------------------------ launch mapping
+
+// Workgroups are dealt to the 8 XCDs of an MI355X round-robin by workgroup ID,
+// and each XCD has its own L2.  Remapping the hardware ID so that XCD k runs a
+// contiguous k-th slice of the logical workgroups keeps spatially adjacent
+// agents (and the neighbour data they share) in one L2.  A bijection on
+// [0, gridDim.x) for any grid size.
+__device__ __forceinline__ uint32_t xcd_block(bool remap) {
+    const uint32_t b = blockIdx.x, G = gridDim.x;
+    if (!remap || G < 16) return b;
+    const uint32_t q = G / 8u, r = G % 8u;
+    const uint32_t xcd = b % 8u, slot = b / 8u;
+    return xcd < r ? xcd * (q + 1u) + slot : r * (q + 1u) + (xcd - r) * q + slot;
 }
 
 // ------------------------------------------------------------------ Philox4x32-10

@@ -49,13 +49,21 @@ hipError_t launch_reorder_ranges(const uint2* hash, uint32_t n, const ReorderArg
 // proximity
 hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, float threshold,
                             const float4* spos, const uint2* hash, const uint32_t* cell_start,
-                            const uint32_t* cell_end, uint32_t n, uint32_t* lists, hipStream_t s);
+                            const uint32_t* cell_end, uint32_t n, uint32_t* lists, uint32_t uniform_max_cells,
+                            uint32_t xcd_remap, hipStream_t s);
+// neighbour lists + steering + integration in one pass (non-KNN lists only)
+hipError_t launch_neighbors_steer(const brt_grid_params& g, const brt_prox_params& p, float threshold,
+                                  const brt_steer_params& sp, float dt, uint32_t step, const float4* spos,
+                                  const float4* sfwd, const float4* ssa, const uint2* hash,
+                                  const uint32_t* cell_start, const uint32_t* cell_end, uint32_t n, uint32_t* lists,
+                                  float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
+                                  uint32_t uniform_max_cells, uint32_t xcd_remap, hipStream_t s);
 
 // opensteer
 hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t step,
                         uint32_t n, const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
                         const uint32_t* lists, float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
-                        float4* force_out, hipStream_t s);
+                        float4* force_out, uint32_t xcd_remap, hipStream_t s);
 hipError_t launch_features(uint32_t K, float rate, uint32_t n, const uint2* hash, const uint32_t* lists,
                            const float4* fin, float4* fout, hipStream_t s);
 

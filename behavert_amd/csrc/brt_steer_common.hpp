@@ -1,0 +1,142 @@
+// brt_steer_common.hpp -- per-neighbour behaviour sums and the per-agent
+// force assembly + integration, shared by the standalone steering kernel
+// (k_steer) and the fused neighbour+steering kernel, so both produce the same
+// bits.
+//
+// Reference: inBoidNeighborhood, calcSeparation/Cohesion/Alignment,
+// xxxsteerForSeek, applySteeringForceSingle
+// (PlugIns/OpenSteerWrapper/OpenSteerWrapper_resources.cu:67-379), the steering
+// kernels and integrator (OpenSteerWrapper_kernel.cu:62-333), the force
+// accumulation order of BehaveRT_Flocking::update
+// (Samples/OpensteerDemo/plugins/BehaveRT_Flocking.cpp:183-196).
+//
+// Cohesion is summed as offsets p_j - p_i (normalize(mean offset)) rather than
+// the reference's normalize(sum p_j / n - p_i): equal in real arithmetic, and
+// free of the FP32 cancellation of a centroid minus a nearby large position.
+#pragma once
+
+#include "brt_device.hpp"
+
+namespace brt {
+
+// inBoidNeighborhood given the (agent, neighbour) pair's dist and cosAngle.
+__device__ __forceinline__ bool inBoid(float dist, float cosAngle, float minR, float maxR, float cosMax) {
+    if (dist < minR) return true;
+    if (dist > maxR) return false;
+    return cosAngle > cosMax;
+}
+
+struct SteerAccum {
+    V3 sSep, sAli, sCoh;
+    uint32_t nSep, nAli, nCoh;
+
+    __device__ __forceinline__ void init() {
+        sSep = v3(0.f, 0.f, 0.f);
+        sAli = v3(0.f, 0.f, 0.f);
+        sCoh = v3(0.f, 0.f, 0.f);
+        nSep = nAli = nCoh = 0;
+    }
+
+    // offset = p_j - p_i, d2 = dot(offset, offset) as the list test computed it
+    __device__ __forceinline__ void add(const brt_steer_params& sp, V3 myFwd, V3 offset, float d2, V3 fwdj) {
+        const float dist = sqrtf(d2);
+        const V3 nrm = offset * (1.0f / dist);
+        const float cosAngle = dot(myFwd, nrm);
+        const float minR = sp.sensing_min_radius;
+        if (sp.separation[0] != 0.0f && inBoid(dist, cosAngle, minR, sp.separation[1], sp.separation[2])) {
+            sSep = sSep + offset / (-d2);
+            ++nSep;
+        }
+        if (sp.alignment[0] != 0.0f && inBoid(dist, cosAngle, minR, sp.alignment[1], sp.alignment[2])) {
+            sAli = sAli + fwdj;
+            ++nAli;
+        }
+        if (sp.cohesion[0] != 0.0f && inBoid(dist, cosAngle, minR, sp.cohesion[1], sp.cohesion[2])) {
+            sCoh = sCoh + offset;
+            ++nCoh;
+        }
+    }
+};
+
+struct AgentOut {
+    float4* pos_out;
+    float4* fwd_out;
+    float4* sa_out;
+    float4* force_out;  // sorted, optional
+    float2* wander;     // by ID, in/out, optional
+};
+
+// Force assembly in the reference's order, then applySteeringForceSingle, then
+// the scatter of the new state to agent-ID order.
+__device__ __forceinline__ void finish_agent(const brt_grid_params& g, const brt_steer_params& sp, float dt,
+                                             uint32_t step, uint32_t i, uint32_t id, V3 myPos, V3 myFwd,
+                                             float mySpeed, V3 mySa, float saW, const SteerAccum& acc,
+                                             const AgentOut& o) {
+    V3 force = v3(0.f, 0.f, 0.f);
+    if (sp.separation[0] != 0.0f) {
+        V3 s = acc.nSep > 0 ? normalize(acc.sSep / (float)acc.nSep) : acc.sSep;
+        force = s * sp.separation[0] + force;
+    }
+    if (sp.alignment[0] != 0.0f) {
+        V3 s = acc.nAli > 0 ? normalize(acc.sAli / (float)acc.nAli - myFwd) : acc.sAli;
+        force = s * sp.alignment[0] + force;
+    }
+    if (sp.cohesion[0] != 0.0f) {
+        V3 s = acc.nCoh > 0 ? normalize(acc.sCoh / (float)acc.nCoh) : acc.sCoh;
+        force = s * sp.cohesion[0] + force;
+    }
+    if (sp.num_targets > 0 && sp.target_weight != 0.0f) {
+        const float* t = sp.targets[id % sp.num_targets];
+        V3 s = steerForSeek(myPos, mySpeed, myFwd, v3(t[0], t[1], t[2]), sp.max_speed);
+        force = s * sp.target_weight + force;
+    }
+    if (sp.wander_weight != 0.0f && o.wander) {
+        uint32_t c[4] = {step, 0u, 0u, 0u};
+        philox4x32_10(c, sp.wander_seed, id);
+        const float u0 = u24(c[0]), u1 = u24(c[1]);
+        const float walk = 12.0f * dt;
+        const float2 wsu = o.wander[id];
+        float ws = wsu.x + ((u0 * 2.0f - 1.0f) * walk);
+        ws = ws < -1.0f ? -1.0f : (ws > 1.0f ? 1.0f : ws);
+        float wu = wsu.y + ((u1 * 2.0f - 1.0f) * walk);
+        wu = wu < -1.0f ? -1.0f : (wu > 1.0f ? 1.0f : wu);
+        o.wander[id] = make_float2(ws, wu);
+        const V3 cr = cross(v3(0.f, 1.f, 0.f), myFwd);
+        const V3 side = dot(cr, cr) == 0.0f ? v3(1.f, 0.f, 0.f) : normalize(cr);
+        const V3 up = cross(myFwd, side);
+        V3 s = side * ws + up * wu;
+        force = s * sp.wander_weight + force;
+    }
+    if (sp.seek_world_center) {
+        const float cx = g.world_center[0], cy = g.world_center[1], cz = g.world_center[2];
+        const float rx = g.world_radius[0], ry = g.world_radius[1], rz = g.world_radius[2];
+        const bool inside = (myPos.x > cx - rx) && (myPos.x < cx + rx) && (myPos.y > cy - ry) &&
+                            (myPos.y < cy + ry) && (myPos.z > cz - rz) && (myPos.z < cz + rz);
+        V3 s = inside ? v3(0.f, 0.f, 0.f) : steerForSeek(myPos, mySpeed, myFwd, v3(cx, cy, cz), sp.max_speed);
+        force = s + force;
+    }
+    if (o.force_out) o.force_out[i] = make_float4(force.x, force.y, force.z, 0.0f);
+
+    if (sp.use_2d_projection) force.y = 0.0f;
+    const V3 adjusted = adjustRawSteeringForce(force, mySpeed, myFwd, sp.max_speed);
+    const V3 clipped = truncateLength(adjusted, sp.max_force);
+    const V3 newAccel = clipped / sp.mass;
+    if (dt > 0.0f) {
+        float smoothRate = clampf(9.0f * dt, 0.15f, 0.4f);
+        smoothRate = clampf(smoothRate, 0.0f, 1.0f);
+        mySa = mySa + (newAccel - mySa) * smoothRate;
+    }
+    V3 newVel = myFwd * mySpeed + mySa * dt;
+    newVel = truncateLength(newVel, sp.max_speed);
+    mySpeed = length(newVel);
+    V3 newPos = myPos + newVel * dt;
+    if (sp.use_2d_projection) newVel.y = 0.0f;
+    const V3 newFwd = normalize(newVel);
+    if (sp.use_2d_projection) newPos.y = 0.5f;
+
+    o.pos_out[id] = make_float4(newPos.x, newPos.y, newPos.z, 1.0f);
+    o.sa_out[id] = make_float4(mySa.x, mySa.y, mySa.z, saW);
+    o.fwd_out[id] = make_float4(newFwd.x, newFwd.y, newFwd.z, mySpeed);
+}
+
+}  // namespace brt

@@ -112,6 +112,15 @@ class Context:
                                                    _ptr(hash_), _ptr(lists), _ptr(fin), _ptr(fout),
                                                    current_stream()))
 
+    # ------------------------------------------------------------ options
+    def set_option(self, option: int, value: int):
+        self.check(self._lib.brt_context_set_option(self.handle, int(option), int(value)))
+
+    def get_option(self, option: int) -> int:
+        v = C.c_int()
+        self.check(self._lib.brt_context_get_option(self.handle, int(option), C.byref(v)))
+        return v.value
+
     # ------------------------------------------------------------ timing
     def phase_timing(self, enable: bool):
         self.check(self._lib.brt_phase_timing_enable(self.handle, 1 if enable else 0))

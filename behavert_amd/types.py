@@ -26,6 +26,9 @@ UNLIMITED = 0xFFFFFFFF
 MAX_TARGETS = 16
 MAX_NEIGHBORS = 252
 NUM_PHASES = 5
+OPT_FUSED_STEP = 1
+OPT_UNIFORM_MAX_CELLS = 2
+OPT_XCD_REMAP = 3
 PHASE_NAMES = ("hash_sort", "reorder_ranges", "neighbors", "steer_integrate", "features")
 
 

@@ -49,7 +49,36 @@ def algorithmic_bytes_per_agent(w) -> dict:
     }
     if w.features is not None and w.feature_rate:
         b["features"] = 4 + 4 * W + 32  # R ID + list, R own feature 16, W 16
+    # fused neighbour lists + steering: the list is written but never re-read
+    b["neighbors_steer"] = 16 + 4 * W + 8 * cpa + 36 + 48
     return b
+
+
+PHASE_KERNELS = {"neighbors": "k_prox<false>", "steer_integrate": "k_steer", "hash_sort": "k_hash_hist+k_onesweep",
+                 "reorder_ranges": "k_reorder_ranges", "features": "k_features",
+                 "neighbors_steer": "k_prox<true>"}
+
+
+def time_phases(ctx, run, first, steps, w, abytes, fused):
+    """Per-phase times (hipEvents on the step's stream) over `steps` steps."""
+    from behavert_amd.types import OPT_FUSED_STEP
+
+    ctx.set_option(OPT_FUSED_STEP, 1 if fused else 0)
+    ctx.phase_timing(True)
+    run(first, steps)
+    ms, nsteps = ctx.phase_times()
+    ctx.phase_timing(False)
+    ctx.set_option(OPT_FUSED_STEP, 1)
+    out = {}
+    for k, v in ms.items():
+        name = "neighbors_steer" if (fused and k == "neighbors") else k
+        if fused and k == "steer_integrate":
+            continue
+        avg = v / max(nsteps, 1)
+        gbs = (abytes[name] * w.n / 1e9) / (avg / 1e3) if avg > 0 and abytes[name] > 0 else 0.0
+        out[name] = {"ms": round(avg, 5), "bytes_per_agent": round(abytes[name], 2), "GB/s": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": PHASE_KERNELS[name]}
+    return out
 
 
 def cpu_baseline(w, sample_steps: int) -> dict:
@@ -109,32 +138,26 @@ def main():
     value = w.n * args.steps / elapsed
 
     abytes = algorithmic_bytes_per_agent(w)
-    total_b = sum(abytes.values())
     phases = {}
     roofline = None
     if not args.no_phases:
-        ctx.phase_timing(True)
-        run(args.warmup + args.steps, args.steps)
-        ms, nsteps = ctx.phase_times()
-        ctx.phase_timing(False)
-        for k, v in ms.items():
-            avg = v / max(nsteps, 1)
-            gbs = (abytes[k] * w.n / 1e9) / (avg / 1e3) if avg > 0 and abytes[k] > 0 else 0.0
-            phases[k] = {"ms": round(avg, 5), "bytes_per_agent": round(abytes[k], 2), "GB/s": round(gbs, 1),
-                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
-        dom = max((k for k in phases if abytes[k] > 0), key=lambda k: phases[k]["ms"])
-        roofline = {"bound": "hbm", "kernel": {"neighbors": "k_neighbors", "steer_integrate": "k_steer",
-                                               "hash_sort": "k_hash_hist+k_onesweep",
-                                               "reorder_ranges": "k_reorder_ranges",
-                                               "features": "k_features"}[dom],
-                    "phase": dom, "achieved": phases[dom]["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": phases[dom]["frac"], "traffic": None,
-                    "bytes_per_agent": round(abytes[dom], 2), "agents_per_launch": w.n}
+        first = args.warmup + args.steps
+        phases["fused_step"] = time_phases(ctx, run, first, args.steps, w, abytes, fused=True)
+        phases["unfused_step"] = time_phases(ctx, run, first + args.steps, args.steps, w, abytes, fused=False)
+        fz = phases["fused_step"]
+        dom = max((k for k in fz if abytes[k] > 0), key=lambda k: fz[k]["ms"])
+        roofline = {"bound": "hbm", "kernel": fz[dom]["kernel"], "phase": dom, "achieved": fz[dom]["GB/s"],
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fz[dom]["frac"], "traffic": None,
+                    "bytes_per_agent": round(abytes[dom], 2), "agents_per_launch": w.n,
+                    "avg_launch_ms": fz[dom]["ms"]}
+        uf = phases["unfused_step"]
         g_keys = ("hash_sort", "reorder_ranges", "neighbors")
-        g_ms = sum(phases[k]["ms"] for k in g_keys)
+        g_ms = sum(uf[k]["ms"] for k in g_keys)
         g_b = sum(abytes[k] for k in g_keys) * w.n
-        phases["grid_neighbor_total"] = {"ms": round(g_ms, 5), "GB/s": round(g_b / 1e9 / (g_ms / 1e3), 1),
-                                         "frac": round(g_b / 1e9 / (g_ms / 1e3) / HBM_PEAK_GBS, 4)}
+        phases["grid_neighbor_phases"] = {"ms": round(g_ms, 5), "GB/s": round(g_b / 1e9 / (g_ms / 1e3), 1),
+                                          "frac": round(g_b / 1e9 / (g_ms / 1e3) / HBM_PEAK_GBS, 4),
+                                          "phases": list(g_keys), "from": "unfused_step"}
+    total_b = sum(abytes[k] for k in ("hash_sort", "reorder_ranges", "neighbors_steer", "features"))
     step_gbs = total_b * w.n / 1e9 / (ms_per_step / 1e3)
 
     cpu = cpu_baseline(w, args.cpu_steps) if args.cpu_steps > 0 else None

@@ -113,6 +113,22 @@ const char* brt_context_last_error(const brt_context* ctx);
 /* Library version string; safe to call without a GPU. */
 const char* brt_version(void);
 
+/* Context options (launch-shape and fusion choices; none changes any result bit).
+ * BRT_OPT_FUSED_STEP: brt_step builds the neighbour lists and runs steering +
+ *   integration in one kernel (default 1; KNN lists always use two passes).
+ * BRT_OPT_UNIFORM_MAX_CELLS: a wave64 whose agents lie in at most this many
+ *   cells scans candidates with wave-uniform (scalar-cache) loads (default 1;
+ *   0 = every wave scans per lane).
+ * BRT_OPT_XCD_REMAP: gather-heavy kernels map contiguous slices of the sorted
+ *   agents to each XCD (one L2 per XCD on MI355X) (default 1).
+ * No counterpart in the reference (its launch shapes are fixed per kernel:
+ * Proximity3DPlugIn.h:133-134, CommonResources.cpp:78-82). */
+#define BRT_OPT_FUSED_STEP 1
+#define BRT_OPT_UNIFORM_MAX_CELLS 2
+#define BRT_OPT_XCD_REMAP 3
+int brt_context_set_option(brt_context* ctx, int option, int value);
+int brt_context_get_option(const brt_context* ctx, int option, int* value);
+
 /* Host-side grid set-up: EnvGrid3DPlugIn::setupGrid
  * (PlugIns/EnvGrid3D/include/EnvGrid3DPlugIn.h:478-508).  No device work. */
 int brt_grid_setup(const float world_center[3], const float world_radius[3],
@@ -239,7 +255,8 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid,
 /* Per-phase timing of brt_step with hipEvents recorded on the step's stream
  * (the reference harness's per-phase timers, BehaveRT_Flocking.cpp:171-204).
  * Phases: 0 hash+sort, 1 reorder+ranges, 2 neighbours, 3 steering+integration,
- * 4 feature update.  brt_phase_times() synchronises the recorded events and
+ * 4 feature update.  With BRT_OPT_FUSED_STEP, phase 2 holds the fused
+ * neighbours+steering kernel and phase 3 is empty.  brt_phase_times() synchronises the recorded events and
  * returns accumulated milliseconds per phase and the number of timed steps. */
 #define BRT_NUM_PHASES 5
 int brt_phase_timing_enable(brt_context* ctx, int enable);

@@ -59,7 +59,7 @@ def lib():
             "orc_neighbors": (None, [GP, PP, P, P, P, P, U, P]),
             "orc_neighbors_brute": (None, [GP, PP, P, P, U, P]),
             "orc_steer_f32": (None, [GP, PP, SP, F, U, U, P, P, P, P, P, P, P, P, P, P]),
-            "orc_steer_f64": (None, [GP, PP, SP, F, U, U, P, P, P, P, P, P, P, P, P, P]),
+            "orc_steer_f64": (None, [GP, PP, SP, F, U, U, P, P, P, P, P, P, P, P, P, P, P]),
             "orc_feature_update_f32": (None, [PP, F, U, P, P, P, P]),
             "orc_feature_update_f64": (None, [PP, F, U, P, P, P, P]),
             "orc_step_f32": (None, [GP, PP, SP, F, F, U, C.POINTER(StepBuffers), P]),
@@ -158,7 +158,7 @@ def neighbors_brute(grid, prox, spos, h) -> np.ndarray:
     return lists
 
 
-def _steer(fn, dtype, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None):
+def _steer(fn, dtype, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None, cond=False):
     n = len(spos)
     spos = np.ascontiguousarray(spos, dtype)
     sfwd = np.ascontiguousarray(sfwd, dtype)
@@ -168,10 +168,16 @@ def _steer(fn, dtype, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wa
     sa = np.zeros((n, 4), dtype)
     force = np.zeros((n, 4), dtype)
     w = None if wander is None else np.array(wander, dtype, copy=True, order="C")
-    fn(C.byref(grid), C.byref(prox), C.byref(steer), float(dt), int(step), n, _p(spos), _p(sfwd), _p(ssa),
-       _p(np.ascontiguousarray(h, np.uint32)), _p(np.ascontiguousarray(lists, np.uint32)), _p(w), _p(pos),
-       _p(fwd), _p(sa), _p(force))
-    return {"pos": pos, "forward": fwd, "smooth_accel": sa, "force": force, "wander": w}
+    args = [C.byref(grid), C.byref(prox), C.byref(steer), float(dt), int(step), n, _p(spos), _p(sfwd), _p(ssa),
+            _p(np.ascontiguousarray(h, np.uint32)), _p(np.ascontiguousarray(lists, np.uint32)), _p(w), _p(pos),
+            _p(fwd), _p(sa), _p(force)]
+    out = {"pos": pos, "forward": fwd, "smooth_accel": sa, "force": force, "wander": w}
+    if cond:
+        k = np.zeros(n, dtype)
+        args.append(_p(k))
+        out["cond"] = k
+    fn(*args)
+    return out
 
 
 def steer_f32(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None):
@@ -179,7 +185,10 @@ def steer_f32(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=Non
 
 
 def steer_f64(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None):
-    return _steer(lib().orc_steer_f64, np.float64, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander)
+    """FP64 evaluation; out["cond"][i] = max over the agent's behaviours of
+    sum|terms| / |sum of terms| (the condition number of its normalisations)."""
+    return _steer(lib().orc_steer_f64, np.float64, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander,
+                  cond=True)
 
 
 def feature_update_f32(prox, rate, h, lists, fin):

@@ -36,8 +36,13 @@
 // Choices this oracle fixes where the reference leaves the last bit open
 // (cutil_math is not in the reference tree):
 //   normalize(v)  = v * (1 / sqrt(dot(v,v)))   (reference: v * rsqrtf(dot))
-//   v / s         = (v.x / s, v.y / s, v.z / s)
+//   v / s         = v * (1 / s)                 (cutil_math's float3 / float)
 //   powf(x, 20)   = ((x^2)^2 * x)^2)^2          (repeated squaring)
+// and one reformulation for accuracy: cohesion is normalize(mean(p_j - p_i))
+// instead of the reference's normalize(sum(p_j)/n - p_i).  The two are equal in
+// real arithmetic; the reference's form cancels catastrophically in FP32 once
+// |p| >> r (a centroid of ~32 positions near 200 minus a position near 200),
+// while the offsets p_j - p_i are exact (Sterbenz) and small.
 // Each is compared against the FP64 evaluation by the tests, not bit-pinned to
 // the reference GPU (which cannot run here).
 
@@ -71,8 +76,12 @@ template <typename R>
 inline V3<R> operator-(V3<R> a, V3<R> b) { return v3<R>(a.x - b.x, a.y - b.y, a.z - b.z); }
 template <typename R>
 inline V3<R> operator*(V3<R> a, R s) { return v3<R>(a.x * s, a.y * s, a.z * s); }
+// cutil_math float3 / float: one reciprocal, three multiplies.
 template <typename R>
-inline V3<R> operator/(V3<R> a, R s) { return v3<R>(a.x / s, a.y / s, a.z / s); }
+inline V3<R> operator/(V3<R> a, R s) {
+    R inv = R(1) / s;
+    return a * inv;
+}
 
 inline float rsqrt_sqrt(float x) { return sqrtf(x); }
 inline double rsqrt_sqrt(double x) { return std::sqrt(x); }
@@ -147,15 +156,22 @@ inline V3<R> adjustRawSteeringForce(V3<R> force, R speed, V3<R> forward, R maxSp
     return vecLimitDeviationAngle(force, cosine, forward);
 }
 
-// OpenSteerWrapper_resources.cu:67-92
+// OpenSteerWrapper_resources.cu:67-92.  A discrete decision: both precisions
+// evaluate it in FP32 on the FP32-rounded state, so the FP64 path differs from
+// the FP32 one only in its continuous arithmetic (a neighbour at d == r exactly
+// is in or out for both).
 template <typename R>
-inline bool inBoidNeighborhood(V3<R> myPos, V3<R> otherPos, R minDist, R maxDist,
-                               R cosMaxAngle, V3<R> forward) {
-    V3<R> offset = otherPos - myPos;
-    R dist = length(offset);
+inline V3<float> f32v(V3<R> v) { return v3<float>((float)v.x, (float)v.y, (float)v.z); }
+
+template <typename R>
+inline bool inBoidNeighborhood(V3<R> myPosR, V3<R> otherPosR, float minDist, float maxDist,
+                               float cosMaxAngle, V3<R> forwardR) {
+    V3<float> myPos = f32v(myPosR), otherPos = f32v(otherPosR), forward = f32v(forwardR);
+    V3<float> offset = otherPos - myPos;
+    float dist = length(offset);
     if (dist < minDist) return true;
     if (dist > maxDist) return false;
-    R cosAngle = dot(forward, normalize(offset));
+    float cosAngle = dot(forward, normalize(offset));
     return cosAngle > cosMaxAngle;
 }
 
@@ -274,6 +290,7 @@ struct SteerArrays {
     R* fwd_out;
     R* sa_out;
     R* force_out;    // sorted, optional
+    R* cond_out;     // sorted, optional: max over behaviours of sum|terms| / |sum|
 };
 
 template <typename R>
@@ -290,55 +307,76 @@ void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& s
     const uint32_t id = A.hash[2 * i + 1];
     const uint32_t* L = A.lists + (size_t)i * A.W;
     const uint32_t count = L[0];
-    const R minR = R(sp.sensing_min_radius);
+    const float minR = sp.sensing_min_radius;
 
     V3<R> force = v3<R>(0, 0, 0);
+    R cond = R(0);
+    auto upd_cond = [&](R mag_sum, V3<R> sum) {
+        R l = length(sum);
+        R k = l > R(0) ? mag_sum / l : R(INFINITY);
+        if (mag_sum > R(0) && k > cond) cond = k;
+    };
 
     // genericSeparationD (OpenSteerWrapper_kernel.cu:86-105), calcSeparation (:166-211)
     if (sp.separation[0] != 0.0f) {
         V3<R> s = v3<R>(0, 0, 0);
+        R mag = R(0);
         uint32_t nb = 0;
         for (uint32_t t = 0; t < count; ++t) {
             uint32_t j = L[1 + t];
             V3<R> other = ld3(A.pos, j);
-            if (inBoidNeighborhood(myPos, other, minR, R(sp.separation[1]), R(sp.separation[2]), myFwd)) {
+            if (inBoidNeighborhood(myPos, other, minR, sp.separation[1], sp.separation[2], myFwd)) {
                 V3<R> offset = other - myPos;
                 R d2 = dot(offset, offset);
                 s = s + offset / (-d2);
+                mag = mag + R(1) / length(offset);
                 ++nb;
             }
         }
-        if (nb > 0) s = normalize(s / R(nb));
+        if (nb > 0) {
+            if (A.cond_out) upd_cond(mag, s);
+            s = normalize(s / R(nb));
+        }
         force = s * R(sp.separation[0]) + force;
     }
     // genericAlignmentD (:158-178), calcAlignment (:253-290)
     if (sp.alignment[0] != 0.0f) {
         V3<R> s = v3<R>(0, 0, 0);
+        R mag = R(0);
         uint32_t nb = 0;
         for (uint32_t t = 0; t < count; ++t) {
             uint32_t j = L[1 + t];
             V3<R> other = ld3(A.pos, j);
-            if (inBoidNeighborhood(myPos, other, minR, R(sp.alignment[1]), R(sp.alignment[2]), myFwd)) {
+            if (inBoidNeighborhood(myPos, other, minR, sp.alignment[1], sp.alignment[2], myFwd)) {
                 s = s + ld3(A.fwd, j);
+                mag = mag + length(ld3(A.fwd, j));
                 ++nb;
             }
         }
-        if (nb > 0) s = normalize(s / R(nb) - myFwd);
+        if (nb > 0) {
+            if (A.cond_out) upd_cond(mag + R(nb) * length(myFwd), s - myFwd * R(nb));
+            s = normalize(s / R(nb) - myFwd);
+        }
         force = s * R(sp.alignment[0]) + force;
     }
     // genericCohesionD (:62-81), calcCohesion (:216-249)
     if (sp.cohesion[0] != 0.0f) {
         V3<R> s = v3<R>(0, 0, 0);
+        R mag = R(0);
         uint32_t nb = 0;
         for (uint32_t t = 0; t < count; ++t) {
             uint32_t j = L[1 + t];
             V3<R> other = ld3(A.pos, j);
-            if (inBoidNeighborhood(myPos, other, minR, R(sp.cohesion[1]), R(sp.cohesion[2]), myFwd)) {
-                s = s + other;
+            if (inBoidNeighborhood(myPos, other, minR, sp.cohesion[1], sp.cohesion[2], myFwd)) {
+                s = s + (other - myPos);
+                mag = mag + length(other - myPos);
                 ++nb;
             }
         }
-        if (nb > 0) s = normalize(s / R(nb) - myPos);
+        if (nb > 0) {
+            if (A.cond_out) upd_cond(mag, s);
+            s = normalize(s / R(nb));
+        }
         force = s * R(sp.cohesion[0]) + force;
     }
     // multi-target seek (build-defined; xxxsteerForSeek toward targets[id % T])
@@ -371,14 +409,17 @@ void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& s
     }
     // genericSeekingWorldCenterD<false> (OpenSteerWrapper_kernel.cu:219-259)
     if (sp.seek_world_center) {
-        const R cx = R(g.world_center[0]), cy = R(g.world_center[1]), cz = R(g.world_center[2]);
-        const R rx = R(g.world_radius[0]), ry = R(g.world_radius[1]), rz = R(g.world_radius[2]);
-        bool inside = (myPos.x > cx - rx) && (myPos.x < cx + rx) && (myPos.y > cy - ry) &&
-                      (myPos.y < cy + ry) && (myPos.z > cz - rz) && (myPos.z < cz + rz);
+        // the box test is a discrete decision: FP32 on the FP32-rounded position
+        const float cx = g.world_center[0], cy = g.world_center[1], cz = g.world_center[2];
+        const float rx = g.world_radius[0], ry = g.world_radius[1], rz = g.world_radius[2];
+        const V3<float> q = f32v(myPos);
+        bool inside = (q.x > cx - rx) && (q.x < cx + rx) && (q.y > cy - ry) && (q.y < cy + ry) &&
+                      (q.z > cz - rz) && (q.z < cz + rz);
         V3<R> s = inside ? v3<R>(0, 0, 0)
-                         : steerForSeek(myPos, mySpeed, myFwd, v3<R>(cx, cy, cz), R(sp.max_speed));
+                         : steerForSeek(myPos, mySpeed, myFwd, v3<R>(R(cx), R(cy), R(cz)), R(sp.max_speed));
         force = s + force;
     }
+    if (A.cond_out) A.cond_out[i] = cond;
     if (A.force_out) {
         A.force_out[4 * i] = force.x;
         A.force_out[4 * i + 1] = force.y;
@@ -432,8 +473,9 @@ void feature_agent(uint32_t i, R rate, const uint32_t* hash, const uint32_t* lis
         const R* q = fin + 4 * (size_t)jid;
         for (int c = 0; c < 4; ++c) s[c] = s[c] + q[c];
     }
+    const R inv = R(1) / R(count);  // cutil float4 / float
     for (int c = 0; c < 4; ++c) {
-        R mean = s[c] / R(count);
+        R mean = s[c] * inv;
         o[c] = f[c] + (mean - f[c]) * rate;
     }
 }
@@ -702,7 +744,7 @@ void orc_steer_f32(const brt_grid_params* g, const brt_prox_params* p, const brt
                    float* pos_out, float* fwd_out, float* sa_out, float* force_out) {
     set_threads();
     SteerArrays<float> A{spos, sfwd, ssa, hash, lists, listStride(p->max_neighbors),
-                         wander, pos_out, fwd_out, sa_out, force_out};
+                         wander, pos_out, fwd_out, sa_out, force_out, nullptr};
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; ++i) steer_agent<float>((uint32_t)i, *g, *s, dt, step, A);
 }
@@ -710,10 +752,10 @@ void orc_steer_f32(const brt_grid_params* g, const brt_prox_params* p, const brt
 void orc_steer_f64(const brt_grid_params* g, const brt_prox_params* p, const brt_steer_params* s,
                    float dt, uint32_t step, uint32_t n, const double* spos, const double* sfwd,
                    const double* ssa, const uint32_t* hash, const uint32_t* lists, double* wander,
-                   double* pos_out, double* fwd_out, double* sa_out, double* force_out) {
+                   double* pos_out, double* fwd_out, double* sa_out, double* force_out, double* cond_out) {
     set_threads();
     SteerArrays<double> A{spos, sfwd, ssa, hash, lists, listStride(p->max_neighbors),
-                          wander, pos_out, fwd_out, sa_out, force_out};
+                          wander, pos_out, fwd_out, sa_out, force_out, cond_out};
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; ++i) steer_agent<double>((uint32_t)i, *g, *s, (double)dt, step, A);
 }

@@ -82,3 +82,38 @@ def test_radius_boundary_exact(ctx, orc):
     prox = ProxParams.make(64, 4.0)
     o = oracle_grid_outputs(orc, grid, prox, st)
     assert first_mismatch(gpu_lists(ctx, grid, prox, o), o["lists"]) is None
+
+
+@pytest.mark.parametrize("uniform_max", [0, 1, 4, 64])
+def test_dense_with_clamped_boundaries_all_scan_modes(ctx, orc, uniform_max):
+    """Dense flock overflowing a small grid on every side: boundary cells hold
+    clamped out-of-grid agents (the cell-box skip must not apply to them); the
+    scan mode (uniform / per lane) must not change a single bit."""
+    from behavert_amd.types import OPT_UNIFORM_MAX_CELLS
+    rng = np.random.default_rng(99)
+    grid = GridParams.direct((0.0,) * 3, (4.0,) * 3, (8,) * 3)
+    xyz = rng.uniform(-3, 35, (40000, 3)).astype(np.float32)
+    xyz[:20] = [[np.nan, 3.0, 3.0]] * 20
+    st = W.make_state(xyz, *W.directions_and_speeds(rng, len(xyz)))
+    prox = ProxParams.make(32, 4.0)
+    o = oracle_grid_outputs(orc, grid, prox, st)
+    old = ctx.get_option(OPT_UNIFORM_MAX_CELLS)
+    try:
+        ctx.set_option(OPT_UNIFORM_MAX_CELLS, uniform_max)
+        assert first_mismatch(gpu_lists(ctx, grid, prox, o), o["lists"]) is None
+    finally:
+        ctx.set_option(OPT_UNIFORM_MAX_CELLS, old)
+
+
+def test_non_power_of_two_cells_offset_origin(ctx, orc):
+    """Cell size 2.5 and an offset origin: cell boxes are not exact in float, so
+    the skip margins are exercised."""
+    rng = np.random.default_rng(123)
+    grid = GridParams.direct((-13.37, 7.25, 100.5), (2.5, 2.5, 2.5), (20, 20, 20))
+    lo = np.array([-13.37, 7.25, 100.5])
+    xyz = (lo + rng.uniform(0, 50, (60000, 3))).astype(np.float32)
+    st = W.make_state(xyz, *W.directions_and_speeds(rng, len(xyz)))
+    for r in (2.5, 2.4999, 1.7):
+        prox = ProxParams.make(24, r)
+        o = oracle_grid_outputs(orc, grid, prox, st)
+        assert first_mismatch(gpu_lists(ctx, grid, prox, o), o["lists"]) is None, r

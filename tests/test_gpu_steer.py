@@ -42,6 +42,9 @@ def test_one_step_bit_exact_vs_fp32_and_within_1e5_of_fp64(ctx, orc, make, recor
     r64 = orc.steer_f64(w.grid, w.prox, w.steer, w.dt, 0, o["sorted_pos"].astype(np.float64),
                         o["sorted_forward"].astype(np.float64), o["sorted_smooth_accel"].astype(np.float64),
                         o["hash"], o["lists"])
+    kappa = r64["cond"]
+    print(f"{w.name}: condition number of the normalisations: median {np.median(kappa):.2f}, "
+          f"p99 {np.quantile(kappa, 0.99):.1f}, max {kappa.max():.3g}")
     for k, floor in FLOORS.items():
         err = floored_rel(got[k][:, :3], r64[k][:, :3], floor)
         record_property(f"{w.name}_{k}_max_err_vs_fp64", float(err.max()))
@@ -150,6 +153,27 @@ def test_step_fused_equals_decomposed_and_oracle_over_10_steps(ctx, orc):
     go = a.grid_outputs()
     assert first_mismatch(go["hash"], hf.hash) is None
     assert first_mismatch(go["lists"], hf.lists) is None
+
+
+@pytest.mark.parametrize("make", [W.plain_4096, W.blobs_1m])
+def test_fused_step_equals_unfused_step(ctx, make):
+    """brt_step's fused neighbour+steering kernel and the two-kernel path give
+    the same bits (lists, hash, state) over 3 steps."""
+    from behavert_amd.types import OPT_FUSED_STEP
+    w = make()
+    res = []
+    for fused in (1, 0):
+        ctx.set_option(OPT_FUSED_STEP, fused)
+        try:
+            fl = run_gpu_steps(ctx, w, 3)
+            res.append((fl.state(), fl.grid_outputs()))
+        finally:
+            ctx.set_option(OPT_FUSED_STEP, 1)
+    (sa, ga), (sb, gb) = res
+    for k in ("pos", "forward", "smooth_accel"):
+        assert first_mismatch(sa[k].view(np.uint32), sb[k].view(np.uint32)) is None, k
+    for k in ("hash", "lists", "cell_start", "cell_end"):
+        assert first_mismatch(ga[k], gb[k]) is None, k
 
 
 def test_ten_step_trajectory_within_1e3_of_fp64(ctx, orc, record_property):
