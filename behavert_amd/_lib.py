@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .types import GridParams, ProxParams, SteerParams, StepBuffers
+from .types import BatchConfig, FlockDesc, GridParams, ProxParams, SteerParams, StepBuffers
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbehavert_amd.so")
 
@@ -41,9 +41,17 @@ SIGNATURES = {
     "brt_osw_steer_integrate": (_I, [_P, _GP, _PP, _SP, _F, _U32, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                      _P, _P]),
     "brt_bc_feature_update": (_I, [_P, _PP, _F, _U32, _P, _P, _P, _P, _P]),
+    "brt_osw_adjoint_step": (_I, [_P, _GP, _PP, _SP, _F, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "brt_step": (_I, [_P, _GP, _PP, _SP, _F, _F, _U32, C.POINTER(StepBuffers), _P]),
     "brt_phase_timing_enable": (_I, [_P, _I]),
     "brt_phase_times": (_I, [_P, C.POINTER(C.c_double), C.POINTER(_U32)]),
+    "brt_batch_create": (_I, [_P, C.POINTER(FlockDesc), _U32, _PP, C.POINTER(_P)]),
+    "brt_batch_destroy": (_I, [_P]),
+    "brt_batch_layout": (_I, [_P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32)]),
+    "brt_batch_set_config": (_I, [_P, C.POINTER(BatchConfig)]),
+    "brt_batch_get_config": (_I, [_P, C.POINTER(BatchConfig)]),
+    "brt_batch_step": (_I, [_P, _SP, _F, _F, _U32, C.POINTER(StepBuffers), _P]),
+    "brt_batch_autotune": (_I, [_P, _SP, _F, C.POINTER(StepBuffers), _U32, C.c_char_p, C.POINTER(BatchConfig), _P]),
 }
 
 _lib = None

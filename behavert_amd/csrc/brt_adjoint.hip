@@ -1,0 +1,342 @@
+// brt_adjoint.hip -- reverse-mode adjoint of one steering + integration step.
+//
+// Forward (one agent i, lists fixed as the forward step built them):
+//   u_s = normalize(mean_j -(p_j-p_i)/|p_j-p_i|^2), u_a = normalize(mean_j f_j - f_i),
+//   u_c = normalize(mean_j (p_j-p_i)), F = w_s u_s + w_a u_a + w_c u_c [+ seeks],
+//   then applySteeringForceSingle (PlugIns/OpenSteerWrapper/OpenSteerWrapper_resources.cu:326-379):
+//   F' = adjustRawSteeringForce(F, s, f), c = truncate(F', maxF), acc = c / m,
+//   a1 = a + (acc - a) r, v0 = f s + a1 dt, v = truncate(v0, maxS),
+//   s1 = |v|, p1 = p + v dt, f1 = normalize(v).
+// Given dL/d(p1, f1, s1, a1) by agent ID this kernel recomputes the forward
+// values of agent i (same arithmetic as the forward kernels), back-propagates
+// through the integrator and the behaviour sums, and scatters dL/d(p, f, s, a)
+// to agent i and to its neighbours j (p_j, f_j appear in i's sums) with
+// hardware float atomics.  The radius/angle predicates, the list membership and
+// the world-box test are piecewise constant and carry no gradient.  dL/dw for
+// the three behaviour weights is reduced per workgroup and then in a fixed order.
+// The reference has no adjoint; the forward it differentiates is the one
+// restated in brt_steer_common.hpp.
+#include "brt_device.hpp"
+#include "brt_internal.hpp"
+#include "brt_steer_common.hpp"
+
+namespace brt {
+
+namespace {
+
+__device__ __forceinline__ V3 normalize_bwd(V3 x, V3 ybar) {
+    const float len = length(x);
+    const V3 y = x * (1.0f / len);
+    return (ybar - y * dot(y, ybar)) * (1.0f / len);
+}
+
+__device__ __forceinline__ V3 truncate_bwd(V3 x, float maxLength, V3 ybar) {
+    const float len = length(x);
+    if (len <= maxLength) return ybar;
+    const V3 u = x * (1.0f / len);
+    return (ybar - u * dot(u, ybar)) * (maxLength / len);
+}
+
+__device__ __forceinline__ void atomic_add3(float4* a, V3 v) {
+    float* p = reinterpret_cast<float*>(a);
+    unsafeAtomicAdd(p + 0, v.x);
+    unsafeAtomicAdd(p + 1, v.y);
+    unsafeAtomicAdd(p + 2, v.z);
+}
+
+struct AdjArgs {
+    brt_grid_params g;
+    brt_steer_params sp;
+    float dt;
+    uint32_t W, n;
+    const float4* spos;
+    const float4* sfwd;
+    const float4* ssa;
+    const uint2* hash;
+    const uint32_t* lists;
+    const float4* gpos_out;
+    const float4* gfwd_out;
+    const float4* gsa_out;
+    float4* gpos_in;
+    float4* gfwd_in;
+    float4* gsa_in;
+    float* partials;  // per workgroup: 3 floats
+    uint32_t xcd_remap;
+};
+
+__device__ __forceinline__ uint32_t list_entry(const uint32_t* row, uint32_t t) { return row[1 + t]; }
+
+__global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
+    __shared__ float s_red[3][4];
+    const uint32_t i = xcd_block(a.xcd_remap != 0) * blockDim.x + threadIdx.x;
+    float wg[3] = {0.f, 0.f, 0.f};
+    const brt_steer_params& sp = a.sp;
+    if (i < a.n && a.hash[i].y < a.n) {
+        const uint32_t id = a.hash[i].y;
+        const float4 P = a.spos[i], Fw = a.sfwd[i], SA = a.ssa[i];
+        const V3 p = xyz(P), f = xyz(Fw), sa = xyz(SA);
+        const float s = Fw.w;
+        const uint32_t* row = a.lists + (size_t)i * a.W;
+        const uint32_t count = row[0] < a.W - 1u ? row[0] : a.W - 1u;
+        const bool behaviours = sp.separation[0] != 0.0f || sp.alignment[0] != 0.0f || sp.cohesion[0] != 0.0f;
+
+        // ---- forward recompute: behaviour sums (list order) -------------------
+        SteerAccum acc;
+        acc.init();
+        if (behaviours)
+            for (uint32_t t = 0; t < count; ++t) {
+                const uint32_t j = list_entry(row, t);
+                if (j >= a.n) continue;
+                const V3 off = xyz(a.spos[j]) - p;
+                acc.add(sp, weights_of(sp), f, off, dot(off, off), xyz(a.sfwd[j]));
+            }
+        V3 xs = v3(0.f, 0.f, 0.f), xa = xs, xc = xs, us = xs, ua = xs, uc = xs;
+        V3 F = v3(0.f, 0.f, 0.f);
+        if (sp.separation[0] != 0.0f) {
+            if (acc.nSep > 0) {
+                xs = acc.sSep / (float)acc.nSep;
+                us = normalize(xs);
+            }
+            F = us * sp.separation[0] + F;
+        }
+        if (sp.alignment[0] != 0.0f) {
+            if (acc.nAli > 0) {
+                xa = acc.sAli / (float)acc.nAli - f;
+                ua = normalize(xa);
+            }
+            F = ua * sp.alignment[0] + F;
+        }
+        if (sp.cohesion[0] != 0.0f) {
+            if (acc.nCoh > 0) {
+                xc = acc.sCoh / (float)acc.nCoh;
+                uc = normalize(xc);
+            }
+            F = uc * sp.cohesion[0] + F;
+        }
+        V3 tgt = v3(0.f, 0.f, 0.f), seekOff = tgt;
+        const bool doTarget = sp.num_targets > 0 && sp.target_weight != 0.0f;
+        if (doTarget) {
+            const float* t = sp.targets[id % sp.num_targets];
+            tgt = v3(t[0], t[1], t[2]);
+            seekOff = tgt - p;
+            F = steerForSeek(p, s, f, tgt, sp.max_speed) * sp.target_weight + F;
+        }
+        bool doCenter = false;
+        V3 centerOff = v3(0.f, 0.f, 0.f);
+        if (sp.seek_world_center) {
+            const brt_grid_params& g = a.g;
+            const bool inside = (p.x > g.world_center[0] - g.world_radius[0]) &&
+                                (p.x < g.world_center[0] + g.world_radius[0]) &&
+                                (p.y > g.world_center[1] - g.world_radius[1]) &&
+                                (p.y < g.world_center[1] + g.world_radius[1]) &&
+                                (p.z > g.world_center[2] - g.world_radius[2]) &&
+                                (p.z < g.world_center[2] + g.world_radius[2]);
+            const V3 c = v3(g.world_center[0], g.world_center[1], g.world_center[2]);
+            doCenter = !inside;
+            centerOff = c - p;
+            F = (inside ? v3(0.f, 0.f, 0.f) : steerForSeek(p, s, f, c, sp.max_speed)) + F;
+        }
+
+        // ---- forward recompute: integrator ------------------------------------
+        const V3 adjF = adjustRawSteeringForce(F, s, f, sp.max_speed);
+        const V3 clipped = truncateLength(adjF, sp.max_force);
+        const V3 accel = clipped / sp.mass;
+        float rate = 0.0f;
+        V3 a1 = sa;
+        if (a.dt > 0.0f) {
+            rate = clampf(clampf(9.0f * a.dt, 0.15f, 0.4f), 0.0f, 1.0f);
+            a1 = sa + (accel - sa) * rate;
+        }
+        const V3 v0 = f * s + a1 * a.dt;
+        const V3 v = truncateLength(v0, sp.max_speed);
+
+        // ---- backward: integrator -------------------------------------------
+        const float4 gP = a.gpos_out[id], gF = a.gfwd_out[id], gA = a.gsa_out[id];
+        const V3 gp1 = xyz(gP), gf1 = xyz(gF), ga1 = xyz(gA);
+        const float gs1 = gF.w;
+        const float vlen = length(v);
+        const V3 f1 = v * (1.0f / vlen);
+        V3 vbar = gp1 * a.dt + normalize_bwd(v, gf1) + f1 * gs1;
+        const V3 v0bar = truncate_bwd(v0, sp.max_speed, vbar);
+        V3 pbar = gp1;
+        V3 fbar = v0bar * s;
+        float sbar = dot(v0bar, f);
+        V3 a1bar = ga1 + v0bar * a.dt;
+        V3 abar = a1bar;
+        V3 accbar = v3(0.f, 0.f, 0.f);
+        if (a.dt > 0.0f) {
+            abar = a1bar * (1.0f - rate);
+            accbar = a1bar * rate;
+        }
+        const V3 clipbar = accbar / sp.mass;
+        V3 Fbar = truncate_bwd(adjF, sp.max_force, clipbar);
+        // adjustRawSteeringForce: identity unless slow and F != 0 and outside the cone
+        const float maxAdj = 0.2f * sp.max_speed;
+        const bool zeroF = F.x == 0.0f && F.y == 0.0f && F.z == 0.0f;
+        if (!(s > maxAdj || zeroF)) {
+            const float range = s / maxAdj;
+            const float c = 1.0f + (-2.0f * pow20(range));
+            const float L = length(F);
+            const V3 d = F / L;
+            if (dot(d, f) < c) {
+                const V3 o = Fbar;  // dL/d(output of the cone limit)
+                const float proj = dot(F, f);
+                const V3 perp = F - f * proj;
+                const float pl = length(perp);
+                const V3 u = perp * (1.0f / pl);
+                const float pd = sqrtf(1.0f - c * c);
+                const V3 inner = f * c + u * pd;
+                const float Lbar = dot(o, inner);
+                const V3 gi = o * L;
+                fbar = fbar + gi * c;
+                const float cbar = dot(gi, f) + dot(gi, u) * (-c / pd);
+                const V3 ubar = gi * pd;
+                const V3 perpbar = (ubar - u * dot(u, ubar)) * (1.0f / pl);
+                Fbar = perpbar - f * dot(f, perpbar) + F * (Lbar / L);
+                fbar = fbar - perpbar * proj - F * dot(f, perpbar);
+                // c = 1 - 2 (s/maxAdj)^20
+                float r19 = 1.0f;
+                for (int k = 0; k < 19; ++k) r19 = r19 * range;
+                sbar = sbar + cbar * (-40.0f * r19 / maxAdj);
+            }
+        }
+        // seeks: d/d(p, f, s) of truncate(target - p, maxS) - f s
+        if (doTarget) {
+            const V3 g = Fbar * sp.target_weight;
+            pbar = pbar - truncate_bwd(seekOff, sp.max_speed, g);
+            fbar = fbar - g * s;
+            sbar = sbar - dot(g, f);
+        }
+        if (doCenter) {
+            pbar = pbar - truncate_bwd(centerOff, sp.max_speed, Fbar);
+            fbar = fbar - Fbar * s;
+            sbar = sbar - dot(Fbar, f);
+        }
+
+        // ---- backward: behaviour sums -------------------------------------------
+        V3 Sbar = v3(0.f, 0.f, 0.f), Abar = Sbar, Cbar = Sbar;
+        if (sp.separation[0] != 0.0f && acc.nSep > 0) {
+            wg[0] = dot(Fbar, us);
+            Sbar = normalize_bwd(xs, Fbar * sp.separation[0]) * (1.0f / (float)acc.nSep);
+        }
+        if (sp.alignment[0] != 0.0f && acc.nAli > 0) {
+            wg[1] = dot(Fbar, ua);
+            const V3 xbar = normalize_bwd(xa, Fbar * sp.alignment[0]);
+            fbar = fbar - xbar;
+            Abar = xbar * (1.0f / (float)acc.nAli);
+        }
+        if (sp.cohesion[0] != 0.0f && acc.nCoh > 0) {
+            wg[2] = dot(Fbar, uc);
+            Cbar = normalize_bwd(xc, Fbar * sp.cohesion[0]) * (1.0f / (float)acc.nCoh);
+        }
+        if (behaviours)
+            for (uint32_t t = 0; t < count; ++t) {
+                const uint32_t j = list_entry(row, t);
+                if (j >= a.n) continue;
+                const uint32_t jid = a.hash[j].y;
+                if (jid >= a.n) continue;
+                const V3 off = xyz(a.spos[j]) - p;
+                const float d2 = dot(off, off);
+                const float dist = sqrtf(d2);
+                const float cosA = dot(f, off * (1.0f / dist));
+                const float minR = sp.sensing_min_radius;
+                V3 offbar = v3(0.f, 0.f, 0.f);
+                bool any = false;
+                if (acc.nSep > 0 && sp.separation[0] != 0.0f &&
+                    inBoid(dist, cosA, minR, sp.separation[1], sp.separation[2])) {
+                    // d/d(off) of -off/|off|^2
+                    const float inv = 1.0f / d2;
+                    offbar = offbar - Sbar * inv + off * (2.0f * dot(off, Sbar) * inv * inv);
+                    any = true;
+                }
+                if (acc.nCoh > 0 && sp.cohesion[0] != 0.0f &&
+                    inBoid(dist, cosA, minR, sp.cohesion[1], sp.cohesion[2])) {
+                    offbar = offbar + Cbar;
+                    any = true;
+                }
+                if (any) {
+                    atomic_add3(&a.gpos_in[jid], offbar);
+                    pbar = pbar - offbar;
+                }
+                if (acc.nAli > 0 && sp.alignment[0] != 0.0f &&
+                    inBoid(dist, cosA, minR, sp.alignment[1], sp.alignment[2]))
+                    atomic_add3(&a.gfwd_in[jid], Abar);
+            }
+        atomic_add3(&a.gpos_in[id], pbar);
+        atomic_add3(&a.gfwd_in[id], fbar);
+        unsafeAtomicAdd(reinterpret_cast<float*>(&a.gfwd_in[id]) + 3, sbar);
+        atomic_add3(&a.gsa_in[id], abar);
+    }
+    // workgroup reduction of dL/dw in a fixed order
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float v = wg[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+        if (lane == 0) s_red[k][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const uint32_t nw = blockDim.x / 64u;
+        float v = 0.0f;
+        for (uint32_t w = 0; w < nw; ++w) v += s_red[threadIdx.x][w];
+        a.partials[(size_t)blockIdx.x * 3 + threadIdx.x] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_zero3(float4* __restrict__ a, float4* __restrict__ b,
+                                               float4* __restrict__ c, uint32_t n) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        a[i] = z;
+        b[i] = z;
+        c[i] = z;
+    }
+}
+
+// Sums the per-workgroup partials in workgroup order (one lane per weight) and
+// adds the result to weight_grad.
+__global__ void k_reduce_wgrad(const float* __restrict__ partials, uint32_t blocks, float* __restrict__ weight_grad) {
+    const uint32_t k = threadIdx.x;
+    if (k >= 3) return;
+    float v = 0.0f;
+    for (uint32_t b = 0; b < blocks; ++b) v += partials[(size_t)b * 3 + k];
+    weight_grad[k] = weight_grad[k] + v;
+}
+
+}  // namespace
+
+hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t n,
+                          const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
+                          const uint32_t* lists, const float4* gpos_out, const float4* gfwd_out,
+                          const float4* gsa_out, float4* gpos_in, float4* gfwd_in, float4* gsa_in,
+                          float* partials, float* weight_grad, uint32_t xcd_remap, hipStream_t s) {
+    const uint32_t blocks = (n + 255) / 256;
+    uint32_t zgrid = blocks < 2048 ? blocks : 2048;
+    hipLaunchKernelGGL(k_zero3, dim3(zgrid), dim3(256), 0, s, gpos_in, gfwd_in, gsa_in, n);
+    AdjArgs a{};
+    a.g = g;
+    a.sp = sp;
+    a.dt = dt;
+    a.W = list_stride(K);
+    a.n = n;
+    a.spos = spos;
+    a.sfwd = sfwd;
+    a.ssa = ssa;
+    a.hash = hash;
+    a.lists = lists;
+    a.gpos_out = gpos_out;
+    a.gfwd_out = gfwd_out;
+    a.gsa_out = gsa_out;
+    a.gpos_in = gpos_in;
+    a.gfwd_in = gfwd_in;
+    a.gsa_in = gsa_in;
+    a.partials = partials;
+    a.xcd_remap = xcd_remap;
+    hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_reduce_wgrad, dim3(1), dim3(64), 0, s, partials, blocks, weight_grad);
+    return hipGetLastError();
+}
+
+}  // namespace brt

@@ -15,26 +15,11 @@
 #include <vector>
 
 #include "../../include/behavert_amd.h"
+#include "brt_host.hpp"
 #include "brt_internal.hpp"
 
-struct brt_context {
-    int device = 0;
-    std::string err;
-    void* tmp = nullptr;        // sort ping-pong buffer (uint2[n])
-    size_t tmp_bytes = 0;
-    uint32_t* meta = nullptr;   // histograms, tile counters, look-back status
-    size_t meta_bytes = 0;
-    bool timing = false;
-    std::vector<std::array<hipEvent_t, BRT_NUM_PHASES + 1>> events;
-    size_t events_used = 0;
-    double phase_ms[BRT_NUM_PHASES] = {0, 0, 0, 0, 0};
-    uint32_t phase_steps = 0;
-    int fused_step = 1;
-    uint32_t uniform_max_cells = 1;
-    uint32_t xcd_remap = 1;
-};
-
-namespace {
+namespace brt {
+namespace host {
 
 int set_err(brt_context* ctx, int code, const char* fmt, ...) {
     if (ctx) {
@@ -52,19 +37,6 @@ int hip_err(brt_context* ctx, hipError_t e, const char* what) {
     if (e == hipErrorOutOfMemory) return set_err(ctx, BRT_ERR_OUT_OF_MEMORY, "%s: %s", what, hipGetErrorString(e));
     return set_err(ctx, BRT_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
-
-#define BRT_HIP(call, what)                              \
-    do {                                                 \
-        hipError_t e_ = (call);                          \
-        if (e_ != hipSuccess) return hip_err(ctx, e_, what); \
-    } while (0)
-
-#define BRT_REQUIRE(cond, ...)                                              \
-    do {                                                                    \
-        if (!(cond)) return set_err(ctx, BRT_ERR_INVALID_ARGUMENT, __VA_ARGS__); \
-    } while (0)
-
-inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 int use_device(brt_context* ctx) {
     int cur = -1;
@@ -165,7 +137,10 @@ void fold_events(brt_context* ctx) {
     ctx->events_used = 0;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace brt
+
+using namespace brt::host;
 
 extern "C" {
 
@@ -191,6 +166,7 @@ int brt_context_destroy(brt_context* ctx) {
         for (auto e : ev) (void)hipEventDestroy(e);
     if (ctx->tmp) (void)hipFree(ctx->tmp);
     if (ctx->meta) (void)hipFree(ctx->meta);
+    if (ctx->partials) (void)hipFree(ctx->partials);
     delete ctx;
     return BRT_OK;
 }
@@ -210,6 +186,9 @@ int brt_context_set_option(brt_context* ctx, int option, int value) {
         case BRT_OPT_XCD_REMAP:
             ctx->xcd_remap = value != 0;
             return BRT_OK;
+        case BRT_OPT_LDS_LISTS:
+            ctx->lds_lists = value != 0;
+            return BRT_OK;
         default:
             return set_err(ctx, BRT_ERR_INVALID_ARGUMENT, "unknown option %d", option);
     }
@@ -226,6 +205,9 @@ int brt_context_get_option(const brt_context* ctx, int option, int* value) {
             return BRT_OK;
         case BRT_OPT_XCD_REMAP:
             *value = (int)ctx->xcd_remap;
+            return BRT_OK;
+        case BRT_OPT_LDS_LISTS:
+            *value = (int)ctx->lds_lists;
             return BRT_OK;
         default:
             return BRT_ERR_INVALID_ARGUMENT;
@@ -392,7 +374,7 @@ int brt_prox_neighbors(brt_context* ctx, const brt_grid_params* grid, const brt_
     BRT_HIP(brt::launch_neighbors(*grid, *prox, brt_radius_threshold(prox->search_radius),
                                   reinterpret_cast<const float4*>(sorted_pos), reinterpret_cast<const uint2*>(hash),
                                   cell_start, cell_end, n, lists, ctx->uniform_max_cells, ctx->xcd_remap,
-                                  (hipStream_t)stream),
+                                  ctx->lds_lists, (hipStream_t)stream),
             "k_neighbors");
     return BRT_OK;
 }
@@ -427,6 +409,56 @@ int brt_osw_steer_integrate(brt_context* ctx, const brt_grid_params* grid, const
                               reinterpret_cast<float4*>(smooth_accel_out), reinterpret_cast<float4*>(force_out),
                               ctx->xcd_remap, (hipStream_t)stream),
             "k_steer");
+    return BRT_OK;
+}
+
+int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
+                         const brt_steer_params* steer, float dt, uint32_t n, const float* sorted_pos,
+                         const float* sorted_forward, const float* sorted_smooth_accel, const uint32_t* hash,
+                         const uint32_t* lists, const float* adj_pos_out, const float* adj_forward_out,
+                         const float* adj_smooth_accel_out, float* adj_pos_in, float* adj_forward_in,
+                         float* adj_smooth_accel_in, float* weight_grad, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_grid(ctx, grid);
+    if (rc) return rc;
+    if ((rc = check_prox(ctx, prox))) return rc;
+    if ((rc = check_steer(ctx, steer))) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (steer->use_2d_projection || steer->wander_weight != 0.0f)
+        return set_err(ctx, BRT_ERR_UNSUPPORTED, "the adjoint supports neither 2-D projection nor wander");
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(sorted_pos && sorted_forward && sorted_smooth_accel && hash && lists && adj_pos_out &&
+                    adj_forward_out && adj_smooth_accel_out && adj_pos_in && adj_forward_in &&
+                    adj_smooth_accel_in && weight_grad,
+                "NULL array");
+    BRT_REQUIRE(aligned16(sorted_pos) && aligned16(sorted_forward) && aligned16(sorted_smooth_accel) &&
+                    aligned16(adj_pos_out) && aligned16(adj_forward_out) && aligned16(adj_smooth_accel_out) &&
+                    aligned16(adj_pos_in) && aligned16(adj_forward_in) && aligned16(adj_smooth_accel_in),
+                "float4 arrays must be 16-byte aligned");
+    BRT_REQUIRE(adj_pos_in != adj_pos_out && adj_forward_in != adj_forward_out &&
+                    adj_smooth_accel_in != adj_smooth_accel_out,
+                "adjoint input and output arrays must differ");
+    if ((rc = use_device(ctx))) return rc;
+    const size_t need = ((size_t)(n + 255) / 256) * 3 * sizeof(float);
+    if (need > ctx->partials_bytes) {
+        if (ctx->partials) BRT_HIP(hipFree(ctx->partials), "hipFree(partials)");
+        ctx->partials = nullptr;
+        ctx->partials_bytes = 0;
+        BRT_HIP(hipMalloc((void**)&ctx->partials, need), "hipMalloc(partials)");
+        ctx->partials_bytes = need;
+    }
+    BRT_HIP(brt::launch_adjoint(*grid, prox->max_neighbors, *steer, dt, n,
+                                reinterpret_cast<const float4*>(sorted_pos),
+                                reinterpret_cast<const float4*>(sorted_forward),
+                                reinterpret_cast<const float4*>(sorted_smooth_accel),
+                                reinterpret_cast<const uint2*>(hash), lists,
+                                reinterpret_cast<const float4*>(adj_pos_out),
+                                reinterpret_cast<const float4*>(adj_forward_out),
+                                reinterpret_cast<const float4*>(adj_smooth_accel_out),
+                                reinterpret_cast<float4*>(adj_pos_in), reinterpret_cast<float4*>(adj_forward_in),
+                                reinterpret_cast<float4*>(adj_smooth_accel_in), ctx->partials, weight_grad,
+                                ctx->xcd_remap, (hipStream_t)stream),
+            "k_adjoint");
     return BRT_OK;
 }
 
@@ -513,7 +545,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
                                             b->lists, reinterpret_cast<float2*>(b->wander),
                                             reinterpret_cast<float4*>(b->pos), reinterpret_cast<float4*>(b->forward),
                                             reinterpret_cast<float4*>(b->smooth_accel), ctx->uniform_max_cells,
-                                            ctx->xcd_remap, s),
+                                            ctx->xcd_remap, ctx->lds_lists, s),
                 "k_prox<fused>");
         if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
         if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
@@ -521,7 +553,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
         // phase 2: neighbour lists
         BRT_HIP(brt::launch_neighbors(*grid, *prox, T, reinterpret_cast<const float4*>(b->sorted_pos),
                                       reinterpret_cast<const uint2*>(b->hash), b->cell_start, b->cell_end, n,
-                                      b->lists, ctx->uniform_max_cells, ctx->xcd_remap, s),
+                                      b->lists, ctx->uniform_max_cells, ctx->xcd_remap, ctx->lds_lists, s),
                 "k_neighbors");
         if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
         // phase 3: fused steering + integration, scattered back to agent-ID order

@@ -87,6 +87,41 @@ __global__ void __launch_bounds__(256) k_hash_hist(brt_grid_params g, const floa
     }
 }
 
+// genCalcHashD over a batch of flocks: each wave hashes up to 64 agents of one
+// flock with that flock's grid, the key offset by the flock's first global
+// cell, so one stable sort orders every flock and keeps each in its own range.
+__global__ void __launch_bounds__(256) k_batch_hash_hist(BatchView b, const float4* __restrict__ pos,
+                                                         uint2* __restrict__ out, uint32_t* __restrict__ hist,
+                                                         SortPlan plan) {
+    __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+    for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) s_hist[t] = 0u;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t wstride = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t gw = blockIdx.x * (blockDim.x >> 6) + wid; gw < b.waves; gw += wstride) {
+        const uint32_t f = __builtin_amdgcn_readfirstlane(b.wave_flock[gw]);
+        const FlockDev& fd = b.flocks[f];
+        const uint32_t i = fd.offset + (gw - fd.wave_base) * 64u + lane;
+        if (i < fd.offset + fd.n) {
+            const float4 p = pos[i];
+            const uint32_t key = fd.cell_base + calcGridHash(calcGridPos(p.x, p.y, p.z, fd.g), fd.g);
+            out[i] = make_uint2(key, i);
+            for (uint32_t q = 0; q < plan.passes; ++q)
+                atomicAdd(&s_hist[q * kRadix + ((key >> plan.shift[q]) & ((1u << plan.nbits[q]) - 1u))], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) {
+        uint32_t v = s_hist[t];
+        if (v) atomicAdd(&hist[t], v);
+    }
+}
+
+__global__ void k_set_word(uint32_t* p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 // Histograms of existing pairs (standalone sort); optionally copies the pairs.
 __global__ void __launch_bounds__(256) k_hist(const uint2* __restrict__ in, uint2* __restrict__ copy_out,
                                               uint32_t n, uint32_t* __restrict__ hist, SortPlan plan) {
@@ -255,8 +290,9 @@ inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t cap) {
 
 }  // namespace
 
-SortPlan make_sort_plan(uint32_t n, uint32_t num_cells) {
+SortPlan make_sort_plan(uint32_t n, uint32_t num_cells, uint32_t items) {
     SortPlan p{};
+    p.items = items == 8u ? 8u : 16u;
     uint32_t bits = 0;
     while (bits < 32 && ((num_cells - 1u) >> bits)) ++bits;
     p.passes = (bits + 7u) / 8u;
@@ -264,7 +300,8 @@ SortPlan make_sort_plan(uint32_t n, uint32_t num_cells) {
         p.shift[q] = 8u * q;
         p.nbits[q] = (bits - 8u * q) < 8u ? (bits - 8u * q) : 8u;
     }
-    p.tiles = (n + kSortTile - 1) / kSortTile;
+    const uint32_t tile = kSortThreads * p.items;
+    p.tiles = (n + tile - 1) / tile;
     return p;
 }
 
@@ -288,6 +325,18 @@ hipError_t launch_hash_hist(const brt_grid_params& g, const float4* pos, uint32_
     return hipGetLastError();
 }
 
+hipError_t launch_batch_hash_hist(const BatchView& b, const float4* pos, uint2* out, uint32_t* meta,
+                                  const SortPlan& plan, hipStream_t s) {
+    uint32_t grid = grid_for(b.waves, 4, 1024);
+    hipLaunchKernelGGL(k_batch_hash_hist, dim3(grid), dim3(256), 0, s, b, pos, out, meta + kMetaHist, plan);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_word(uint32_t* p, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(64), 0, s, p, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_hist(const uint2* in, uint2* copy_out, uint32_t n, uint32_t* meta, const SortPlan& plan,
                        hipStream_t s) {
     uint32_t grid = grid_for(n, 256, 1024);
@@ -298,9 +347,12 @@ hipError_t launch_hist(const uint2* in, uint2* copy_out, uint32_t n, uint32_t* m
 hipError_t launch_onesweep(const uint2* in, uint2* out, uint32_t n, uint32_t pass, const SortPlan& plan,
                            uint32_t* meta, hipStream_t s) {
     uint32_t* status = meta + kMetaStatus + (size_t)pass * plan.tiles * kRadix;
-    hipLaunchKernelGGL(k_onesweep<kSortItems>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n,
-                       plan.shift[pass], plan.nbits[pass], meta + kMetaHist + pass * kRadix, status,
-                       meta + kMetaCounters + pass);
+    if (plan.items == 8u)
+        hipLaunchKernelGGL(k_onesweep<8>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n, plan.shift[pass],
+                           plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass);
+    else
+        hipLaunchKernelGGL(k_onesweep<16>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n, plan.shift[pass],
+                           plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass);
     return hipGetLastError();
 }
 

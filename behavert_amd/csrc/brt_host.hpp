@@ -1,0 +1,61 @@
+// brt_host.hpp -- the library context and the host-side helpers shared by the
+// C ABI translation units (status codes, validation, scratch management).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "../../include/behavert_amd.h"
+#include "brt_internal.hpp"
+
+struct brt_context {
+    int device = 0;
+    std::string err;
+    void* tmp = nullptr;        // sort ping-pong buffer (uint2[n])
+    size_t tmp_bytes = 0;
+    uint32_t* meta = nullptr;   // histograms, tile counters, look-back status
+    size_t meta_bytes = 0;
+    float* partials = nullptr;  // adjoint: per-workgroup weight-gradient partials
+    size_t partials_bytes = 0;
+    bool timing = false;
+    std::vector<std::array<hipEvent_t, BRT_NUM_PHASES + 1>> events;
+    size_t events_used = 0;
+    double phase_ms[BRT_NUM_PHASES] = {0, 0, 0, 0, 0};
+    uint32_t phase_steps = 0;
+    int fused_step = 1;
+    uint32_t uniform_max_cells = 1;
+    uint32_t xcd_remap = 1;
+    uint32_t lds_lists = 1;
+};
+
+namespace brt {
+namespace host {
+
+int set_err(brt_context* ctx, int code, const char* fmt, ...);
+int hip_err(brt_context* ctx, hipError_t e, const char* what);
+int use_device(brt_context* ctx);
+int check_grid(brt_context* ctx, const brt_grid_params* g);
+int check_n(brt_context* ctx, uint32_t n);
+int check_prox(brt_context* ctx, const brt_prox_params* p);
+int check_steer(brt_context* ctx, const brt_steer_params* s);
+int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan);
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace host
+}  // namespace brt
+
+#define BRT_HIP(call, what)                              \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return hip_err(ctx, e_, what); \
+    } while (0)
+
+#define BRT_REQUIRE(cond, ...)                                              \
+    do {                                                                    \
+        if (!(cond)) return set_err(ctx, BRT_ERR_INVALID_ARGUMENT, __VA_ARGS__); \
+    } while (0)
+

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #include "../../include/behavert_amd.h"
@@ -20,11 +21,12 @@ constexpr uint32_t kMetaStatus = 1088;       // uint32 status[passes][tiles][256
 struct SortPlan {
     uint32_t passes;
     uint32_t tiles;
+    uint32_t items;  // keys per thread: 16 (tile 4096) or 8 (tile 2048)
     uint32_t shift[kMaxPasses];
     uint32_t nbits[kMaxPasses];
 };
 
-SortPlan make_sort_plan(uint32_t n, uint32_t num_cells);
+SortPlan make_sort_plan(uint32_t n, uint32_t num_cells, uint32_t items = kSortItems);
 size_t sort_meta_words(const SortPlan& p);
 
 // envgrid
@@ -50,14 +52,14 @@ hipError_t launch_reorder_ranges(const uint2* hash, uint32_t n, const ReorderArg
 hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, float threshold,
                             const float4* spos, const uint2* hash, const uint32_t* cell_start,
                             const uint32_t* cell_end, uint32_t n, uint32_t* lists, uint32_t uniform_max_cells,
-                            uint32_t xcd_remap, hipStream_t s);
+                            uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s);
 // neighbour lists + steering + integration in one pass (non-KNN lists only)
 hipError_t launch_neighbors_steer(const brt_grid_params& g, const brt_prox_params& p, float threshold,
                                   const brt_steer_params& sp, float dt, uint32_t step, const float4* spos,
                                   const float4* sfwd, const float4* ssa, const uint2* hash,
                                   const uint32_t* cell_start, const uint32_t* cell_end, uint32_t n, uint32_t* lists,
                                   float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
-                                  uint32_t uniform_max_cells, uint32_t xcd_remap, hipStream_t s);
+                                  uint32_t uniform_max_cells, uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s);
 
 // opensteer
 hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t step,
@@ -67,6 +69,50 @@ hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_pa
 hipError_t launch_features(uint32_t K, float rate, uint32_t n, const uint2* hash, const uint32_t* lists,
                            const float4* fin, float4* fout, hipStream_t s);
 
+// adjoint
+hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t n,
+                          const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
+                          const uint32_t* lists, const float4* gpos_out, const float4* gfwd_out,
+                          const float4* gsa_out, float4* gpos_in, float4* gfwd_in, float4* gsa_in,
+                          float* partials, float* weight_grad, uint32_t xcd_remap, hipStream_t s);
+
 inline uint32_t list_stride(uint32_t K) { return 4u * (K / 4u + 1u); }
+
+// ---------------------------------------------------------------- batches
+// One flock of a batch as the kernels see it.  Flocks are concatenated: flock f
+// owns agent IDs and sorted indices [offset, offset + n) and global cell keys
+// [cell_base, cell_base + g.num_cells).  Waves are assigned per flock (a wave
+// never spans two flocks): flock f owns waves [wave_base, wave_base + ceil(n/64)).
+struct alignas(16) FlockDev {
+    brt_grid_params g;
+    float slack[3];      // cell-box rounding margins of g
+    uint32_t offset, n, cell_base, wave_base;
+    float w[3];          // separation / alignment / cohesion weights
+    uint32_t pad[4];
+};
+static_assert(sizeof(FlockDev) == 128, "FlockDev is two 64-byte scalar loads");
+
+struct BatchView {
+    const FlockDev* flocks;
+    const uint32_t* wave_flock;  // flock of each wave
+    uint32_t waves;              // total waves
+    const uint32_t* step;        // device step counter (wander), may be NULL
+};
+
+// per-axis cell-box rounding margins (see the neighbour kernels)
+inline void cell_slack(const brt_grid_params& g, float out[3]) {
+    for (int ax = 0; ax < 3; ++ax)
+        out[ax] = 1e-5f * (fabsf(g.world_origin[ax]) + (float)g.grid_size[ax] * g.cell_size[ax]);
+}
+
+hipError_t launch_batch_hash_hist(const BatchView& b, const float4* pos, uint2* out, uint32_t* meta,
+                                  const SortPlan& plan, hipStream_t s);
+hipError_t launch_set_word(uint32_t* p, uint32_t v, hipStream_t s);
+hipError_t launch_batch_neighbors_steer(const BatchView& b, uint32_t n, const brt_prox_params& p, float threshold,
+                                        const brt_steer_params& sp, float dt, const float4* spos, const float4* sfwd,
+                                        const float4* ssa, const uint2* hash, const uint32_t* cell_start,
+                                        const uint32_t* cell_end, uint32_t* lists, float2* wander, float4* pos_out,
+                                        float4* fwd_out, float4* sa_out, uint32_t block, uint32_t uniform_max_cells,
+                                        uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s);
 
 }  // namespace brt

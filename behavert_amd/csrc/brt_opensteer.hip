@@ -44,7 +44,8 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
     const uint4* L = reinterpret_cast<const uint4*>(lists + (size_t)i * W);
     const uint4 w0 = L[0];
     const uint32_t count = w0.x < W - 1u ? w0.x : W - 1u;
-    const bool any = sp.separation[0] != 0.0f || sp.alignment[0] != 0.0f || sp.cohesion[0] != 0.0f;
+    const Weights wts = weights_of(sp);
+    const bool any = wts.s != 0.0f || wts.a != 0.0f || wts.c != 0.0f;
 
     SteerAccum acc;
     acc.init();
@@ -68,12 +69,12 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
                 const uint32_t e = 4u * q + t;
                 if (e >= 1u && e <= count && jj[t] < n) {
                     const V3 offset = xyz(qp[t]) - myPos;
-                    acc.add(sp, myFwd, offset, dot(offset, offset), xyz(qf[t]));
+                    acc.add(sp, wts, myFwd, offset, dot(offset, offset), xyz(qf[t]));
                 }
             }
         }
     }
-    finish_agent(g, sp, dt, step, i, id, myPos, myFwd, Fw.w, xyz(SA), SA.w, acc, o);
+    finish_agent(g, sp, wts, dt, step, i, id, id, myPos, myFwd, Fw.w, xyz(SA), SA.w, acc, o);
 }
 
 __global__ void __launch_bounds__(256) k_features(uint32_t W, float rate, uint32_t n, const uint2* __restrict__ hash,

@@ -51,11 +51,11 @@ __device__ __forceinline__ float4 cload(cfloat4_ptr p, uint32_t i) {
 }
 
 struct ProxArgs {
-    brt_grid_params g;
-    uint32_t K, W, n;
-    float T;          // exact squared-radius threshold
-    float Tskip;      // T * (1 + 1e-5): cell-skip threshold on the box distance
-    float slack[3];   // per-axis widening of cell boxes for rounding
+    brt_grid_params g;  // single-grid launches
+    uint32_t K, W, n;   // n = all agents of the launch (a batch: of every flock)
+    float T;            // exact squared-radius threshold
+    float Tskip;        // T * (1 + 1e-5): cell-skip threshold on the box distance
+    float slack[3];     // per-axis widening of cell boxes for rounding (single grid)
     uint32_t uniform_max_cells;
     uint32_t xcd_remap;
     const float4* spos;
@@ -70,13 +70,62 @@ struct ProxArgs {
     float dt;
     uint32_t step;
     AgentOut o;
+    // batches: per-flock grids, weights and wave ranges
+    BatchView b;
 };
+
+// The agents one wave works on and the grid they live in (wave-uniform).
+struct Seg {
+    brt_grid_params g;
+    float slack[3];
+    uint32_t cell_base;  // global key of the grid's cell 0
+    uint32_t row0;       // sorted index of the wave's lane 0
+    uint32_t end;        // one past the segment's last sorted index
+    uint32_t id_base;    // agent ID of the segment's first agent
+    Weights w;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ bool wave_segment(const ProxArgs& a, uint32_t gw, Seg& sg) {
+    if (!BATCH) {
+        sg.g = a.g;
+        sg.slack[0] = a.slack[0];
+        sg.slack[1] = a.slack[1];
+        sg.slack[2] = a.slack[2];
+        sg.cell_base = 0;
+        sg.row0 = gw * 64u;
+        sg.end = a.n;
+        sg.id_base = 0;
+        sg.w = weights_of(a.sp);
+        return sg.row0 < a.n;
+    }
+    if (gw >= a.b.waves) return false;
+    const uint32_t f = __builtin_amdgcn_readfirstlane(a.b.wave_flock[gw]);
+    const FlockDev& fd = a.b.flocks[f];
+    sg.g = fd.g;
+    sg.slack[0] = fd.slack[0];
+    sg.slack[1] = fd.slack[1];
+    sg.slack[2] = fd.slack[2];
+    sg.cell_base = fd.cell_base;
+    sg.row0 = fd.offset + (gw - fd.wave_base) * 64u;
+    sg.end = fd.offset + fd.n;
+    sg.id_base = fd.offset;
+    sg.w = Weights{fd.w[0], fd.w[1], fd.w[2]};
+    return true;
+}
 
 struct Lane {
     float px, py, pz;
     uint32_t i;
     uint32_t cnt;
-    uint32_t* row;  // LDS list row
+    uint32_t* row;       // the agent's list row: staged in LDS, or in global memory
+    uint32_t h0, h1, h2;  // register-chunk lists: entries 0..2 (chunk 0, with the count)
+    uint32_t b0, b1, b2;  // register-chunk lists: entries of the pending 16-byte chunk
+    // squared distances from the agent to the boxes of the cells one below
+    // (lo) and one above (hi) its own on each axis, each gap shrunk by the
+    // rounding slack: the box distance of neighbour (dx,dy,dz) is the sum of
+    // the three selected terms (offset 0 contributes 0)
+    float lo2[3], hi2[3];
 };
 
 __device__ __forceinline__ bool interior(int cx, int cy, int cz, const brt_grid_params& g) {
@@ -84,64 +133,118 @@ __device__ __forceinline__ bool interior(int cx, int cy, int cz, const brt_grid_
            cz <= (int)g.grid_size[2] - 2;
 }
 
-// squared distance from (px,py,pz) to the box of cell (cx,cy,cz), each axis gap
-// shrunk by the rounding slack
-__device__ __forceinline__ float box_d2(const ProxArgs& a, float px, float py, float pz, int cx, int cy, int cz) {
-    const brt_grid_params& g = a.g;
-    float lo, gx, gy, gz;
-    lo = g.world_origin[0] + (float)cx * g.cell_size[0];
-    gx = fmaxf(fmaxf(lo - px, px - (lo + g.cell_size[0])) - a.slack[0], 0.0f);
-    lo = g.world_origin[1] + (float)cy * g.cell_size[1];
-    gy = fmaxf(fmaxf(lo - py, py - (lo + g.cell_size[1])) - a.slack[1], 0.0f);
-    lo = g.world_origin[2] + (float)cz * g.cell_size[2];
-    gz = fmaxf(fmaxf(lo - pz, pz - (lo + g.cell_size[2])) - a.slack[2], 0.0f);
-    return (gx * gx + gy * gy) + gz * gz;
+__device__ __forceinline__ void cell_gaps(const Seg& sg, Lane& L, I3 gp) {
+    const float p[3] = {L.px, L.py, L.pz};
+    const int c[3] = {gp.x, gp.y, gp.z};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const float lo = sg.g.world_origin[ax] + (float)c[ax] * sg.g.cell_size[ax];
+        const float f = p[ax] - lo;
+        const float gl = fmaxf(f - sg.slack[ax], 0.0f);
+        const float gh = fmaxf((sg.g.cell_size[ax] - f) - sg.slack[ax], 0.0f);
+        L.lo2[ax] = gl * gl;
+        L.hi2[ax] = gh * gh;
+    }
 }
 
-__device__ __forceinline__ void accept(Lane& L, uint32_t j) {
-    L.row[1 + L.cnt] = j;
-    ++L.cnt;
+__device__ __forceinline__ float gap2(const Lane& L, int ax, int d) {
+    return d < 0 ? L.lo2[ax] : (d > 0 ? L.hi2[ax] : 0.0f);
+}
+
+// Appends sorted index c to the lane's list when take.  LDS lists: one LDS
+// word per entry.  Register-chunk lists: the row's 16-byte chunk q holds words
+// 4q..4q+3, i.e. entries 4q-1..4q+2; entries are gathered in registers and a
+// chunk is stored to global memory when its last entry arrives (chunk 0, which
+// holds the count, and the final partial chunk are stored by finish_list).
+template <bool LDSL>
+__device__ __forceinline__ void append(Lane& L, uint32_t c, bool take) {
+    if (LDSL) {
+        if (take) L.row[1 + L.cnt] = c;
+    } else if (take) {
+        const uint32_t k = L.cnt;
+        if (k < 3u) {
+            L.h0 = k == 0u ? c : L.h0;
+            L.h1 = k == 1u ? c : L.h1;
+            L.h2 = k == 2u ? c : L.h2;
+        } else {
+            const uint32_t s = (k - 3u) & 3u;
+            if (s == 3u) {
+                reinterpret_cast<uint4*>(L.row)[(k - 3u) / 4u + 1u] = make_uint4(L.b0, L.b1, L.b2, c);
+            } else {
+                L.b0 = s == 0u ? c : L.b0;
+                L.b1 = s == 1u ? c : L.b1;
+                L.b2 = s == 2u ? c : L.b2;
+            }
+        }
+    }
+    L.cnt += take ? 1u : 0u;
+}
+
+// Count and padding (LDS lists), or the chunks not yet stored (register-chunk
+// lists: the pending partial chunk, the all-padding chunks, then chunk 0).
+template <bool LDSL>
+__device__ __forceinline__ void finish_list(Lane& L, uint32_t W) {
+    if (LDSL) {
+        L.row[0] = L.cnt;
+        for (uint32_t q = L.cnt + 1; q < W; ++q) L.row[q] = BRT_LIST_PAD;
+        return;
+    }
+    const uint32_t P = BRT_LIST_PAD, k = L.cnt;
+    uint4* row = reinterpret_cast<uint4*>(L.row);
+    uint32_t q = (k > 3u ? (k - 3u) / 4u : 0u) + 1u;
+    const uint32_t rem = k > 3u ? (k - 3u) & 3u : 0u;
+    if (4u * q < W) {
+        row[q] = make_uint4(rem > 0u ? L.b0 : P, rem > 1u ? L.b1 : P, rem > 2u ? L.b2 : P, P);
+        ++q;
+    }
+    for (; 4u * q < W; ++q) row[q] = make_uint4(P, P, P, P);
+    row[0] = make_uint4(k, k > 0u ? L.h0 : P, k > 1u ? L.h1 : P, k > 2u ? L.h2 : P);
+}
+
+// Tests candidate c (position q) for lane L and appends it when it passes,
+// without branching.
+template <bool LDSL>
+__device__ __forceinline__ void test_candidate(const ProxArgs& a, Lane& L, uint32_t c, uint32_t lim, float4 q) {
+    const float rx = q.x - L.px, ry = q.y - L.py, rz = q.z - L.pz;
+    const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
+    const float d2 = (xx + yy) + zz;
+    const bool take = (c < lim) & (c != L.i) & !(d2 > a.T) & (L.cnt < a.K);
+    append<LDSL>(L, c, take);
 }
 
 // Uniform mode: every active lane shares cell (ux,uy,uz); control flow and
 // candidate indices are wave-uniform.
-__device__ void scan_uniform(const ProxArgs& a, Lane& L, int ux, int uy, int uz) {
-    const brt_grid_params& g = a.g;
+template <bool LDSL>
+__device__ __forceinline__ void scan_uniform(const ProxArgs& a, const Seg& sg, Lane& L, int ux, int uy, int uz) {
+    const brt_grid_params& g = sg.g;
     cfloat4_ptr cpos = (cfloat4_ptr)a.spos;
     cuint_ptr cstart = (cuint_ptr)a.cell_start;
     cuint_ptr cend = (cuint_ptr)a.cell_end;
     const uint32_t maxBodies = g.max_bodies_per_cell;
+#pragma unroll
     for (int dx = -1; dx <= 1; ++dx) {
         for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
             for (int dz = -1; dz <= 1; ++dz) {
                 const int cx = ux + dx, cy = uy + dy, cz = uz + dz;
                 if (!inGrid(cx, cy, cz, g)) continue;
-                const uint32_t key = cellKey(cx, cy, cz, g);
+                const uint32_t key = sg.cell_base + cellKey(cx, cy, cz, g);
                 const uint32_t s = cstart[key];
                 if (s == BRT_EMPTY_CELL) continue;
                 const uint32_t e = cend[key];
                 uint32_t lim = (e - s) > maxBodies ? s + maxBodies : e;
                 lim = lim < a.n ? lim : a.n;
                 if (interior(cx, cy, cz, g)) {
-                    const bool need = L.cnt < a.K && !(box_d2(a, L.px, L.py, L.pz, cx, cy, cz) > a.Tskip);
+                    const float bd = (gap2(L, 0, dx) + gap2(L, 1, dy)) + gap2(L, 2, dz);
+                    const bool need = (L.cnt < a.K) & !(bd > a.Tskip);
                     if (__ballot(need) == 0) continue;
                 }
                 for (uint32_t j = s; j < lim; j += 4) {
                     float4 q[4];
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const uint32_t jj = (j + t < lim) ? j + t : lim - 1;
-                        q[t] = cload(cpos, jj);
-                    }
+                    for (int t = 0; t < 4; ++t) q[t] = cload(cpos, (j + t < lim) ? j + t : lim - 1);
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        if (j + t < lim && j + t != L.i && L.cnt < a.K) {
-                            const float rx = q[t].x - L.px, ry = q[t].y - L.py, rz = q[t].z - L.pz;
-                            const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
-                            const float d2 = (xx + yy) + zz;
-                            if (!(d2 > a.T)) accept(L, j + t);
-                        }
-                    }
+                    for (int t = 0; t < 4; ++t) test_candidate<LDSL>(a, L, j + t, lim, q[t]);
                     if (__ballot(L.cnt < a.K) == 0) return;
                 }
             }
@@ -149,64 +252,71 @@ __device__ void scan_uniform(const ProxArgs& a, Lane& L, int ux, int uy, int uz)
     }
 }
 
-// Vector mode: each lane walks its own canonical stream.
-__device__ void scan_vector(const ProxArgs& a, Lane& L, int gx, int gy, int gz) {
-    const brt_grid_params& g = a.g;
+// Vector mode: each lane walks its own canonical stream.  The nine cell ranges
+// of one x-offset slab are loaded together (one memory latency per slab rather
+// than two per cell).
+template <bool LDSL>
+__device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, Lane& L, I3 gp) {
+    const brt_grid_params& g = sg.g;
     const uint32_t maxBodies = g.max_bodies_per_cell;
+#pragma unroll
     for (int dx = -1; dx <= 1; ++dx) {
-        for (int dy = -1; dy <= 1; ++dy) {
-            for (int dz = -1; dz <= 1; ++dz) {
-                const int cx = gx + dx, cy = gy + dy, cz = gz + dz;
-                if (!inGrid(cx, cy, cz, g)) continue;
-                const uint32_t key = cellKey(cx, cy, cz, g);
-                const uint32_t s = a.cell_start[key];
-                if (s == BRT_EMPTY_CELL) continue;
-                if (interior(cx, cy, cz, g) && box_d2(a, L.px, L.py, L.pz, cx, cy, cz) > a.Tskip) continue;
-                const uint32_t e = a.cell_end[key];
-                uint32_t lim = (e - s) > maxBodies ? s + maxBodies : e;
-                lim = lim < a.n ? lim : a.n;
-                for (uint32_t j = s; j < lim; j += 4) {
-                    float4 q[4];
+        const int cx = gp.x + dx;
+        uint32_t st[9], en[9];
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const uint32_t jj = (j + t < lim) ? j + t : lim - 1;
-                        q[t] = a.spos[jj];
-                    }
+        for (int k = 0; k < 9; ++k) {
+            const int cy = gp.y + k / 3 - 1, cz = gp.z + k % 3 - 1;
+            const bool ok = inGrid(cx, cy, cz, g);
+            const uint32_t key = sg.cell_base + (ok ? cellKey(cx, cy, cz, g) : 0u);
+            st[k] = a.cell_start[key];
+            en[k] = a.cell_end[key];
+            st[k] = ok ? st[k] : BRT_EMPTY_CELL;
+        }
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        if (j + t < lim && j + t != L.i) {
-                            const float rx = q[t].x - L.px, ry = q[t].y - L.py, rz = q[t].z - L.pz;
-                            const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
-                            const float d2 = (xx + yy) + zz;
-                            if (!(d2 > a.T)) {
-                                accept(L, j + t);
-                                if (L.cnt == a.K) return;
-                            }
-                        }
-                    }
-                }
+        for (int k = 0; k < 9; ++k) {
+            const int dy = k / 3 - 1, dz = k % 3 - 1;
+            const uint32_t s = st[k];
+            if (s == BRT_EMPTY_CELL) continue;
+            const float bd = (gap2(L, 0, dx) + gap2(L, 1, dy)) + gap2(L, 2, dz);
+            if (bd > a.Tskip && interior(cx, gp.y + dy, gp.z + dz, g)) continue;
+            const uint32_t e = en[k];
+            uint32_t lim = (e - s) > maxBodies ? s + maxBodies : e;
+            lim = lim < a.n ? lim : a.n;
+            for (uint32_t j = s; j < lim; j += 4) {
+                float4 q[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) q[t] = a.spos[(j + t < lim) ? j + t : lim - 1];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) test_candidate<LDSL>(a, L, j + t, lim, q[t]);
+                if (L.cnt >= a.K) return;
             }
         }
     }
 }
 
-template <bool FUSE>
+template <bool FUSE, bool BATCH, bool LDSL>
 __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
-    extern __shared__ uint32_t s_list[];  // blockDim.x * W
+    extern __shared__ uint32_t s_list[];  // blockDim.x * W with LDS lists, else unused
     const uint32_t tid = threadIdx.x;
-    const uint32_t B = blockDim.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t blk = xcd_block(a.xcd_remap != 0);
+    const uint32_t gw = blk * (blockDim.x >> 6) + wid;
+    Seg sg;
+    const bool wave_live = wave_segment<BATCH>(a, gw, sg);
     Lane L;
-    L.i = blk * B + tid;
+    L.i = sg.row0 + lane;
     L.cnt = 0;
-    L.row = s_list + tid * a.W;
-    const bool alive = L.i < a.n;
+    const bool alive = wave_live && L.i < sg.end;
+    L.row = LDSL ? s_list + tid * a.W : a.lists + (size_t)(alive ? L.i : 0u) * a.W;
+    L.h0 = L.h1 = L.h2 = L.b0 = L.b1 = L.b2 = BRT_LIST_PAD;
     float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
     if (alive) P = a.spos[L.i];
     L.px = P.x;
     L.py = P.y;
     L.pz = P.z;
-    I3 gp = calcGridPos(P.x, P.y, P.z, a.g);
+    I3 gp = calcGridPos(P.x, P.y, P.z, sg.g);
+    cell_gaps(sg, L, gp);
 
     // wave-level mode choice: can the wave's lanes be covered by a few cells?
     const uint64_t alive_mask = __ballot(alive);
@@ -229,25 +339,27 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
             const int uz = __builtin_amdgcn_readlane(gp.z, leader);
             const bool mine = alive && gp.x == ux && gp.y == uy && gp.z == uz;
             pending &= ~__ballot(mine);
-            if (mine) scan_uniform(a, L, ux, uy, uz);
+            if (mine) scan_uniform<LDSL>(a, sg, L, ux, uy, uz);
         }
     } else if (alive) {
-        scan_vector(a, L, gp.x, gp.y, gp.z);
+        scan_vector<LDSL>(a, sg, L, gp);
     }
 
-    L.row[0] = L.cnt;
-    for (uint32_t q = L.cnt + 1; q < a.W; ++q) L.row[q] = BRT_LIST_PAD;
-    __syncthreads();
-    const uint32_t row0 = blk * B;
-    const uint32_t rows = (a.n - row0) < B ? (a.n - row0) : B;
-    const uint32_t words4 = rows * a.W / 4u;
-    uint4* dst = reinterpret_cast<uint4*>(a.lists + (size_t)row0 * a.W);
-    const uint4* src = reinterpret_cast<const uint4*>(s_list);
-    for (uint32_t w = tid; w < words4; w += B) dst[w] = src[w];
+    if (LDSL || alive) finish_list<LDSL>(L, a.W);
+    if (LDSL) __syncthreads();
+    // LDS lists: each wave writes its own rows, 16 bytes per lane per store
+    if (LDSL && wave_live) {
+        const uint32_t rows = (sg.end - sg.row0) < 64u ? (sg.end - sg.row0) : 64u;
+        const uint32_t words4 = rows * a.W / 4u;
+        uint4* dst = reinterpret_cast<uint4*>(a.lists + (size_t)sg.row0 * a.W);
+        const uint4* src = reinterpret_cast<const uint4*>(s_list + (size_t)(wid * 64u) * a.W);
+        for (uint32_t w = lane; w < words4; w += 64u) dst[w] = src[w];
+    }
 
-    // fused steering: walk the finished list from LDS (all lanes in step, four
-    // neighbour gathers in flight), then integrate.  Same order and operations
-    // as k_steer over the written list.
+    // fused steering: walk the finished list (from LDS, or the row this lane
+    // just stored), all lanes in step with four neighbour gathers in flight,
+    // then integrate.  Same order and operations as k_steer over the written
+    // list.
     if (FUSE && alive) {
         const uint32_t id = a.hash[L.i].y;
         if (id < a.n) {
@@ -256,7 +368,8 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
             const V3 myPos = v3(P.x, P.y, P.z), myFwd = xyz(Fw);
             SteerAccum acc;
             acc.init();
-            if (a.sp.separation[0] != 0.0f || a.sp.alignment[0] != 0.0f || a.sp.cohesion[0] != 0.0f) {
+            const Weights wts = sg.w;
+            if (wts.s != 0.0f || wts.a != 0.0f || wts.c != 0.0f) {
                 for (uint32_t t = 0; t < L.cnt; t += 4) {
                     float4 q[4], f[4];
 #pragma unroll
@@ -269,12 +382,14 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
                     for (int u = 0; u < 4; ++u) {
                         if (t + u < L.cnt) {
                             const V3 offset = xyz(q[u]) - myPos;
-                            acc.add(a.sp, myFwd, offset, dot(offset, offset), xyz(f[u]));
+                            acc.add(a.sp, wts, myFwd, offset, dot(offset, offset), xyz(f[u]));
                         }
                     }
                 }
             }
-            finish_agent(a.g, a.sp, a.dt, a.step, L.i, id, myPos, myFwd, Fw.w, xyz(SA), SA.w, acc, a.o);
+            const uint32_t step = (BATCH && a.b.step) ? *a.b.step : a.step;
+            finish_agent(sg.g, a.sp, wts, a.dt, step, L.i, id, id - sg.id_base, myPos, myFwd, Fw.w, xyz(SA), SA.w,
+                         acc, a.o);
         }
     }
 }
@@ -368,8 +483,7 @@ ProxArgs make_args(const brt_grid_params& g, const brt_prox_params& p, float thr
     // per-axis widening of 1e-5 of the grid's coordinate range, both far above
     // the few-ulp rounding of positions, cell bounds and d2
     a.Tskip = threshold * 1.00001f;
-    for (int ax = 0; ax < 3; ++ax)
-        a.slack[ax] = 1e-5f * (fabsf(g.world_origin[ax]) + (float)g.grid_size[ax] * g.cell_size[ax]);
+    cell_slack(g, a.slack);
     a.uniform_max_cells = uniform_max_cells;
     a.xcd_remap = xcd_remap;
     a.spos = spos;
@@ -385,7 +499,7 @@ ProxArgs make_args(const brt_grid_params& g, const brt_prox_params& p, float thr
 hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, float threshold,
                             const float4* spos, const uint2* hash, const uint32_t* cell_start,
                             const uint32_t* cell_end, uint32_t n, uint32_t* lists, uint32_t uniform_max_cells,
-                            uint32_t xcd_remap, hipStream_t s) {
+                            uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s) {
     const uint32_t K = p.max_neighbors;
     const uint32_t W = list_stride(K);
     if (p.use_knn) {
@@ -398,7 +512,10 @@ hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, 
     }
     ProxArgs a = make_args(g, p, threshold, spos, hash, cell_start, cell_end, n, lists, uniform_max_cells, xcd_remap);
     const uint32_t B = prox_block(W);
-    hipLaunchKernelGGL(k_prox<false>, dim3((n + B - 1) / B), dim3(B), (size_t)B * W * 4u, s, a);
+    if (lds_lists)
+        hipLaunchKernelGGL((k_prox<false, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * W * 4u, s, a);
+    else
+        hipLaunchKernelGGL((k_prox<false, false, false>), dim3((n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -407,7 +524,7 @@ hipError_t launch_neighbors_steer(const brt_grid_params& g, const brt_prox_param
                                   const float4* sfwd, const float4* ssa, const uint2* hash,
                                   const uint32_t* cell_start, const uint32_t* cell_end, uint32_t n, uint32_t* lists,
                                   float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
-                                  uint32_t uniform_max_cells, uint32_t xcd_remap, hipStream_t s) {
+                                  uint32_t uniform_max_cells, uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s) {
     ProxArgs a = make_args(g, p, threshold, spos, hash, cell_start, cell_end, n, lists, uniform_max_cells, xcd_remap);
     a.sfwd = sfwd;
     a.ssa = ssa;
@@ -416,7 +533,35 @@ hipError_t launch_neighbors_steer(const brt_grid_params& g, const brt_prox_param
     a.step = step;
     a.o = AgentOut{pos_out, fwd_out, sa_out, nullptr, wander};
     const uint32_t B = prox_block(a.W);
-    hipLaunchKernelGGL(k_prox<true>, dim3((n + B - 1) / B), dim3(B), (size_t)B * a.W * 4u, s, a);
+    if (lds_lists)
+        hipLaunchKernelGGL((k_prox<true, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.W * 4u, s, a);
+    else
+        hipLaunchKernelGGL((k_prox<true, false, false>), dim3((n + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_neighbors_steer(const BatchView& b, uint32_t n, const brt_prox_params& p, float threshold,
+                                        const brt_steer_params& sp, float dt, const float4* spos, const float4* sfwd,
+                                        const float4* ssa, const uint2* hash, const uint32_t* cell_start,
+                                        const uint32_t* cell_end, uint32_t* lists, float2* wander, float4* pos_out,
+                                        float4* fwd_out, float4* sa_out, uint32_t block, uint32_t uniform_max_cells,
+                                        uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s) {
+    brt_grid_params g0{};
+    ProxArgs a = make_args(g0, p, threshold, spos, hash, cell_start, cell_end, n, lists, uniform_max_cells, xcd_remap);
+    a.sfwd = sfwd;
+    a.ssa = ssa;
+    a.sp = sp;
+    a.dt = dt;
+    a.step = 0;
+    a.o = AgentOut{pos_out, fwd_out, sa_out, nullptr, wander};
+    a.b = b;
+    const uint32_t B = block ? block : prox_block(a.W);
+    const uint32_t wpb = B / 64u;
+    if (lds_lists)
+        hipLaunchKernelGGL((k_prox<true, true, true>), dim3((b.waves + wpb - 1) / wpb), dim3(B), (size_t)B * a.W * 4u,
+                           s, a);
+    else
+        hipLaunchKernelGGL((k_prox<true, true, false>), dim3((b.waves + wpb - 1) / wpb), dim3(B), 0, s, a);
     return hipGetLastError();
 }
 

@@ -26,6 +26,15 @@ __device__ __forceinline__ bool inBoid(float dist, float cosAngle, float minR, f
     return cosAngle > cosMax;
 }
 
+// Behaviour weights (separation, alignment, cohesion): the steer params' own,
+// or a flock's in a batch.
+struct Weights {
+    float s, a, c;
+};
+__device__ __forceinline__ Weights weights_of(const brt_steer_params& sp) {
+    return Weights{sp.separation[0], sp.alignment[0], sp.cohesion[0]};
+}
+
 struct SteerAccum {
     V3 sSep, sAli, sCoh;
     uint32_t nSep, nAli, nCoh;
@@ -38,20 +47,21 @@ struct SteerAccum {
     }
 
     // offset = p_j - p_i, d2 = dot(offset, offset) as the list test computed it
-    __device__ __forceinline__ void add(const brt_steer_params& sp, V3 myFwd, V3 offset, float d2, V3 fwdj) {
+    __device__ __forceinline__ void add(const brt_steer_params& sp, Weights w, V3 myFwd, V3 offset, float d2,
+                                        V3 fwdj) {
         const float dist = sqrtf(d2);
         const V3 nrm = offset * (1.0f / dist);
         const float cosAngle = dot(myFwd, nrm);
         const float minR = sp.sensing_min_radius;
-        if (sp.separation[0] != 0.0f && inBoid(dist, cosAngle, minR, sp.separation[1], sp.separation[2])) {
+        if (w.s != 0.0f && inBoid(dist, cosAngle, minR, sp.separation[1], sp.separation[2])) {
             sSep = sSep + offset / (-d2);
             ++nSep;
         }
-        if (sp.alignment[0] != 0.0f && inBoid(dist, cosAngle, minR, sp.alignment[1], sp.alignment[2])) {
+        if (w.a != 0.0f && inBoid(dist, cosAngle, minR, sp.alignment[1], sp.alignment[2])) {
             sAli = sAli + fwdj;
             ++nAli;
         }
-        if (sp.cohesion[0] != 0.0f && inBoid(dist, cosAngle, minR, sp.cohesion[1], sp.cohesion[2])) {
+        if (w.c != 0.0f && inBoid(dist, cosAngle, minR, sp.cohesion[1], sp.cohesion[2])) {
             sCoh = sCoh + offset;
             ++nCoh;
         }
@@ -67,32 +77,35 @@ struct AgentOut {
 };
 
 // Force assembly in the reference's order, then applySteeringForceSingle, then
-// the scatter of the new state to agent-ID order.
-__device__ __forceinline__ void finish_agent(const brt_grid_params& g, const brt_steer_params& sp, float dt,
-                                             uint32_t step, uint32_t i, uint32_t id, V3 myPos, V3 myFwd,
+// the scatter of the new state to agent-ID order.  `id` indexes the state
+// arrays; `lid` (the ID within the agent's flock, equal to id outside batches)
+// picks the seek target and keys the wander stream.
+__device__ __forceinline__ void finish_agent(const brt_grid_params& g, const brt_steer_params& sp, Weights w,
+                                             float dt, uint32_t step, uint32_t i, uint32_t id, uint32_t lid,
+                                             V3 myPos, V3 myFwd,
                                              float mySpeed, V3 mySa, float saW, const SteerAccum& acc,
                                              const AgentOut& o) {
     V3 force = v3(0.f, 0.f, 0.f);
-    if (sp.separation[0] != 0.0f) {
+    if (w.s != 0.0f) {
         V3 s = acc.nSep > 0 ? normalize(acc.sSep / (float)acc.nSep) : acc.sSep;
-        force = s * sp.separation[0] + force;
+        force = s * w.s + force;
     }
-    if (sp.alignment[0] != 0.0f) {
+    if (w.a != 0.0f) {
         V3 s = acc.nAli > 0 ? normalize(acc.sAli / (float)acc.nAli - myFwd) : acc.sAli;
-        force = s * sp.alignment[0] + force;
+        force = s * w.a + force;
     }
-    if (sp.cohesion[0] != 0.0f) {
+    if (w.c != 0.0f) {
         V3 s = acc.nCoh > 0 ? normalize(acc.sCoh / (float)acc.nCoh) : acc.sCoh;
-        force = s * sp.cohesion[0] + force;
+        force = s * w.c + force;
     }
     if (sp.num_targets > 0 && sp.target_weight != 0.0f) {
-        const float* t = sp.targets[id % sp.num_targets];
+        const float* t = sp.targets[lid % sp.num_targets];
         V3 s = steerForSeek(myPos, mySpeed, myFwd, v3(t[0], t[1], t[2]), sp.max_speed);
         force = s * sp.target_weight + force;
     }
     if (sp.wander_weight != 0.0f && o.wander) {
         uint32_t c[4] = {step, 0u, 0u, 0u};
-        philox4x32_10(c, sp.wander_seed, id);
+        philox4x32_10(c, sp.wander_seed, lid);
         const float u0 = u24(c[0]), u1 = u24(c[1]);
         const float walk = 12.0f * dt;
         const float2 wsu = o.wander[id];

@@ -18,7 +18,8 @@ import numpy as np
 import torch
 
 from ._lib import BrtError, lib
-from .types import NUM_PHASES, PHASE_NAMES, GridParams, ProxParams, SteerParams, StepBuffers, list_stride
+from .types import (NUM_PHASES, PHASE_NAMES, BatchConfig, FlockDesc, GridParams, ProxParams, SteerParams,
+                    StepBuffers, list_stride)
 
 
 def _ptr(t: torch.Tensor | None) -> int | None:
@@ -132,6 +133,35 @@ class Context:
         return {PHASE_NAMES[p]: ms[p] for p in range(NUM_PHASES)}, steps.value
 
 
+def _alloc_buffers(obj, state: dict, num_cells: int, max_neighbors: int, features, wander, device):
+    """The per-agent state (ID order), its sorted copies, the hash, cell ranges and
+    neighbour lists of one step, as torch device tensors on obj."""
+    n = state["pos"].shape[0]
+    obj.n = n
+    dev = torch.device(device)
+    f32 = dict(dtype=torch.float32, device=dev)
+    obj.pos = torch.from_numpy(np.ascontiguousarray(state["pos"], np.float32)).to(dev)
+    obj.forward = torch.from_numpy(np.ascontiguousarray(state["forward"], np.float32)).to(dev)
+    obj.smooth_accel = torch.from_numpy(np.ascontiguousarray(state["smooth_accel"], np.float32)).to(dev)
+    obj.sorted_pos = torch.empty((n, 4), **f32)
+    obj.sorted_forward = torch.empty((n, 4), **f32)
+    obj.sorted_smooth_accel = torch.empty((n, 4), **f32)
+    u32 = dict(dtype=torch.int32, device=dev)  # uint32 bit patterns
+    obj.hash = torch.empty((n, 2), **u32)
+    obj.cell_start = torch.empty((num_cells,), **u32)
+    obj.cell_end = torch.empty((num_cells,), **u32)
+    obj.W = list_stride(max_neighbors)
+    obj.lists = torch.empty((n, obj.W), **u32)
+    obj.features = None
+    obj.features_out = None
+    if features is not None:
+        obj.features = torch.from_numpy(np.ascontiguousarray(features, np.float32)).to(dev)
+        obj.features_out = torch.empty_like(obj.features)
+    obj.wander = None
+    if wander is not None:
+        obj.wander = torch.from_numpy(np.ascontiguousarray(wander, np.float32)).to(dev)
+
+
 class Flock:
     """Device buffers of one flock (the features the reference's plug-ins allocate:
     EnvGrid3DPlugIn.h:157-173, Proximity3DPlugIn.h:167-174, OpenSteerWrapperPlugIn.h:153-178)."""
@@ -142,30 +172,7 @@ class Flock:
         self.grid = grid
         self.prox = prox
         self.steer = steer
-        n = state["pos"].shape[0]
-        self.n = n
-        dev = torch.device(device)
-        f32 = dict(dtype=torch.float32, device=dev)
-        self.pos = torch.from_numpy(np.ascontiguousarray(state["pos"], np.float32)).to(dev)
-        self.forward = torch.from_numpy(np.ascontiguousarray(state["forward"], np.float32)).to(dev)
-        self.smooth_accel = torch.from_numpy(np.ascontiguousarray(state["smooth_accel"], np.float32)).to(dev)
-        self.sorted_pos = torch.empty((n, 4), **f32)
-        self.sorted_forward = torch.empty((n, 4), **f32)
-        self.sorted_smooth_accel = torch.empty((n, 4), **f32)
-        u32 = dict(dtype=torch.int32, device=dev)  # uint32 bit patterns
-        self.hash = torch.empty((n, 2), **u32)
-        self.cell_start = torch.empty((grid.num_cells,), **u32)
-        self.cell_end = torch.empty((grid.num_cells,), **u32)
-        self.W = list_stride(prox.max_neighbors)
-        self.lists = torch.empty((n, self.W), **u32)
-        self.features = None
-        self.features_out = None
-        if features is not None:
-            self.features = torch.from_numpy(np.ascontiguousarray(features, np.float32)).to(dev)
-            self.features_out = torch.empty_like(self.features)
-        self.wander = None
-        if wander is not None:
-            self.wander = torch.from_numpy(np.ascontiguousarray(wander, np.float32)).to(dev)
+        _alloc_buffers(self, state, grid.num_cells, prox.max_neighbors, features, wander, device)
 
     def buffers(self) -> StepBuffers:
         b = StepBuffers()
@@ -233,3 +240,80 @@ class Flock:
                 "sorted_pos": self.sorted_pos.cpu().numpy(),
                 "sorted_forward": self.sorted_forward.cpu().numpy(),
                 "sorted_smooth_accel": self.sorted_smooth_accel.cpu().numpy()}
+
+
+class Batch(Flock):
+    """Independent flocks advanced by one batched step (brt_batch_*): the state
+    arrays are the flocks' arrays concatenated in order; flock f owns agent IDs
+    and sorted indices [offsets[f], offsets[f] + sizes[f]) and cell keys
+    [cell_bases[f], cell_bases[f] + grids[f].num_cells)."""
+
+    def __init__(self, ctx: Context, grids, sizes, weights, prox: ProxParams, steer: SteerParams, state: dict,
+                 features: np.ndarray | None = None, wander: np.ndarray | None = None, device="cuda"):
+        self.ctx = ctx
+        self.prox = prox
+        self.steer = steer
+        self.grid = None
+        F = len(grids)
+        descs = (FlockDesc * F)()
+        for f in range(F):
+            descs[f].grid = grids[f]
+            descs[f].n = int(sizes[f])
+            for k in range(3):
+                descs[f].weights[k] = float(weights[f][k])
+        h = C.c_void_p()
+        ctx.check(ctx._lib.brt_batch_create(ctx.handle, descs, F, C.byref(prox), C.byref(h)))
+        self.handle = h
+        tn, tc = C.c_uint32(), C.c_uint32()
+        offs, cbs = (C.c_uint32 * F)(), (C.c_uint32 * F)()
+        ctx.check(ctx._lib.brt_batch_layout(h, C.byref(tn), C.byref(tc), offs, cbs))
+        self.total_cells = int(tc.value)
+        self.offsets = np.array(offs[:], np.int64)
+        self.cell_bases = np.array(cbs[:], np.int64)
+        self.sizes = np.asarray(sizes, np.int64)
+        if state["pos"].shape[0] != tn.value:
+            raise ValueError(f"state holds {state['pos'].shape[0]} agents, the flocks {tn.value}")
+        _alloc_buffers(self, state, self.total_cells, prox.max_neighbors, features, wander, device)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx._lib.brt_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self) -> BatchConfig:
+        c = BatchConfig()
+        self.ctx.check(self.ctx._lib.brt_batch_get_config(self.handle, C.byref(c)))
+        return c
+
+    def set_config(self, **kw):
+        c = self.config()
+        for k, v in kw.items():
+            setattr(c, k, int(v))
+        self.ctx.check(self.ctx._lib.brt_batch_set_config(self.handle, C.byref(c)))
+
+    def autotune(self, dt: float, steps_per_trial: int = 2, cache_path: str | None = None) -> BatchConfig:
+        c = BatchConfig()
+        b = self.buffers()
+        self.ctx.check(self.ctx._lib.brt_batch_autotune(
+            self.handle, C.byref(self.steer), float(dt), C.byref(b), int(steps_per_trial),
+            None if cache_path is None else cache_path.encode(), C.byref(c), current_stream()))
+        return c
+
+    def step(self, dt: float, step: int, feature_rate: float = 0.0):
+        b = self.buffers()
+        self.ctx.check(self.ctx._lib.brt_batch_step(self.handle, C.byref(self.steer), float(feature_rate), float(dt),
+                                                    int(step), C.byref(b), current_stream()))
+        if self.features is not None and feature_rate != 0.0:
+            self.features, self.features_out = self.features_out, self.features
+
+    def flock_slice(self, f: int) -> slice:
+        return slice(int(self.offsets[f]), int(self.offsets[f] + self.sizes[f]))
+
+    def cell_slice(self, f: int, num_cells: int) -> slice:
+        return slice(int(self.cell_bases[f]), int(self.cell_bases[f] + num_cells))

@@ -29,6 +29,7 @@ NUM_PHASES = 5
 OPT_FUSED_STEP = 1
 OPT_UNIFORM_MAX_CELLS = 2
 OPT_XCD_REMAP = 3
+OPT_LDS_LISTS = 4
 PHASE_NAMES = ("hash_sort", "reorder_ranges", "neighbors", "steer_integrate", "features")
 
 
@@ -156,6 +157,26 @@ class StepBuffers(C.Structure):
         ("features_out", C.c_void_p),
         ("wander", C.c_void_p),
     ]
+
+
+class FlockDesc(C.Structure):
+    """One flock of a batch: its own grid, agent count and behaviour weights."""
+    _fields_ = [("grid", GridParams), ("n", C.c_uint32), ("weights", C.c_float * 3)]
+
+
+class BatchConfig(C.Structure):
+    """Tunables of the batched step (brt_batch_config)."""
+    _fields_ = [
+        ("prox_block", C.c_uint32),
+        ("uniform_max_cells", C.c_uint32),
+        ("xcd_remap", C.c_uint32),
+        ("sort_items", C.c_uint32),
+        ("lds_lists", C.c_uint32),
+        ("use_graph", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 def list_stride(k: int) -> int:

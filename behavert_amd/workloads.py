@@ -187,6 +187,50 @@ def batched_flock_sizes(seed: int = 0, flocks: int = 256, lo: int = 1024, hi: in
     return np.round(np.exp(g.uniform(math.log(lo), math.log(hi), flocks))).astype(np.int64)
 
 
+@dataclass
+class BatchWorkload:
+    """Independent flocks advanced together; flock f is sizes[f] agents uniform in
+    a cube of side (n/density)^(1/3) with its own grid and weights."""
+    name: str
+    sizes: np.ndarray
+    sides: np.ndarray
+    grids: list
+    weights: np.ndarray      # (F, 3) float32
+    prox: ProxParams
+    steer: SteerParams
+    dt: float
+    steps: int
+    states: list             # per-flock state dicts
+
+    @property
+    def n(self) -> int:
+        return int(self.sizes.sum())
+
+    def concat_state(self) -> dict:
+        return {k: np.concatenate([st[k] for st in self.states]) for k in ("pos", "forward", "smooth_accel")}
+
+    def describe(self) -> dict:
+        return {"workload": self.name, "flocks": len(self.sizes), "n": self.n,
+                "flock_sizes": [int(self.sizes.min()), int(self.sizes.max())],
+                "density": 0.5, "cell": CELL, "radius": self.prox.search_radius,
+                "max_neighbors": self.prox.max_neighbors, "steps": self.steps, "dt": self.dt}
+
+
+def batched_flocks(seed: int = 0, flocks: int = 256, lo: int = 1024, hi: int = 65536,
+                   density: float = 0.5, steps: int = 200) -> BatchWorkload:
+    """256 flocks with sizes log-uniform in [1024, 65536], each in its own cube at
+    0.5 agents per unit^3, cell 4, r 4, K 32, weights U[0.5, 1.5] per flock and
+    behaviour, 200 steps.  Flock f's state uses seed 1000*(seed+1)+f."""
+    sizes = batched_flock_sizes(seed, flocks, lo, hi)
+    sides = (sizes / density) ** (1.0 / 3.0)
+    grids = [cube_grid(float(sd)) for sd in sides]
+    g = rng(seed + 7)
+    weights = g.uniform(0.5, 1.5, (flocks, 3)).astype(np.float32)
+    states = [uniform_flock(int(n), float(sd), 1000 * (seed + 1) + f) for f, (n, sd) in enumerate(zip(sizes, sides))]
+    return BatchWorkload("batched_256", sizes, sides, grids, weights, ProxParams.make(32, RADIUS), plain_steer(),
+                         DT, steps, states)
+
+
 WORKLOADS = {
     "plain_4096": plain_4096,
     "blobs_1m": blobs_1m,

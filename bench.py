@@ -99,6 +99,123 @@ def cpu_baseline(w, sample_steps: int) -> dict:
             "ms_per_step": 1e3 * dt / sample_steps}
 
 
+def adjoint_ratio(steps=None, every=None) -> dict:
+    """Differentiable rollout (65,536 boids in [0,160)^3, K 32): time of the plain
+    forward step vs a forward step + its adjoint step, and of the whole
+    checkpointed rollout (forward with checkpoints, then a backward that re-runs
+    each segment and walks the adjoint) vs the plain forward rollout."""
+    import torch
+
+    from behavert_amd import workloads as W
+    from behavert_amd.autograd import Rollout
+    from behavert_amd.engine import Context
+
+    w = W.differentiable_65k()
+    T = steps or w.steps
+    K = every or w.extra["checkpoint_every"]
+    ctx = Context(0)
+    ro = Rollout(ctx, w.grid, w.prox, w.steer, w.n, w.dt, T, K)
+    dev = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
+    p0, f0, a0 = dev(w.state["pos"]), dev(w.state["forward"]), dev(w.state["smooth_accel"])
+    wts = list(w.extra["start_weights"])
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    # plain forward rollout (the same steps, no checkpoints kept beyond the few copies)
+    ro.forward(wts, p0, f0, a0)
+    torch.cuda.synchronize()
+    e0, e1, e2 = ev(), ev(), ev()
+    e0.record()
+    ro.forward(wts, p0, f0, a0)
+    e1.record()
+    g = torch.zeros_like(p0)
+    g[:, :3] = 1.0 / w.n
+    z = torch.zeros_like(p0)
+    ro.backward(g, z, z)
+    e2.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1)
+    bwd_ms = e1.elapsed_time(e2)
+    # one step: forward step vs adjoint kernel alone, over the last recorded segment
+    import ctypes as C
+
+    from behavert_amd import lib
+
+    L = lib()
+    e3, e4 = ev(), ev()
+    reps = 20
+    e3.record()
+    for _ in range(reps):
+        go, gi = ro.g[0], ro.g[1]
+        ctx.check(L.brt_osw_adjoint_step(ctx.handle, C.byref(ro.grid), C.byref(ro.prox), C.byref(ro.steer), ro.dt,
+                                         ro.n, ro.r_spos[0].data_ptr(), ro.r_sfwd[0].data_ptr(),
+                                         ro.r_ssa[0].data_ptr(), ro.r_hash[0].data_ptr(), ro.r_lists[0].data_ptr(),
+                                         go[0].data_ptr(), go[1].data_ptr(), go[2].data_ptr(), gi[0].data_ptr(),
+                                         gi[1].data_ptr(), gi[2].data_ptr(), ro.wgrad.data_ptr(),
+                                         torch.cuda.current_stream().cuda_stream))
+    e4.record()
+    torch.cuda.synchronize()
+    adj_ms = e3.elapsed_time(e4) / reps
+    step_ms = fwd_ms / T
+    ctx.close()
+    return {"workload": "differentiable_65k", "n": w.n, "steps": T, "checkpoint_every": K,
+            "forward_step_ms": round(step_ms, 5), "adjoint_step_ms": round(adj_ms, 5),
+            "step_ratio": round((step_ms + adj_ms) / step_ms, 3),
+            "forward_rollout_ms": round(fwd_ms, 3), "backward_rollout_ms": round(bwd_ms, 3),
+            "rollout_ratio": round((fwd_ms + bwd_ms) / fwd_ms, 3)}
+
+
+def batched(steps=20, loop_steps=3) -> dict:
+    """256 flocks (sizes log-uniform in [1024, 65536], 0.5 agents per unit^3, K 32,
+    per-flock weights): the autotuned, graph-replayed batched step vs brt_step
+    looped over the same flocks."""
+    import ctypes as C
+
+    import torch
+
+    from behavert_amd import workloads as W
+    from behavert_amd.engine import Batch, Context, Flock
+    from behavert_amd.types import SteerParams
+
+    w = W.batched_flocks()
+    ctx = Context(0)
+    b = Batch(ctx, w.grids, w.sizes, w.weights, w.prox, w.steer, w.concat_state())
+    t0 = time.perf_counter()
+    cfg = b.autotune(w.dt, 2)
+    tune_s = time.perf_counter() - t0
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for i in range(k):
+            fn(i)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / k
+
+    timed(lambda i: b.step(w.dt, i), 3)
+    batch_ms = timed(lambda i: b.step(w.dt, 3 + i), steps)
+    flocks = []
+    for f in range(len(w.sizes)):
+        sp = SteerParams()
+        C.memmove(C.byref(sp), C.byref(w.steer), C.sizeof(SteerParams))
+        sp.separation[0], sp.alignment[0], sp.cohesion[0] = (float(x) for x in w.weights[f])
+        flocks.append(Flock(ctx, w.grids[f], w.prox, sp, w.states[f]))
+
+    def loop(i):
+        for fl in flocks:
+            fl.step(w.dt, i)
+
+    timed(loop, 1)
+    loop_ms = timed(loop, loop_steps)
+    b.close()
+    ctx.close()
+    return {**w.describe(), "tuned": cfg.as_dict(), "tune_s": round(tune_s, 3),
+            "batched_ms_per_step": round(batch_ms, 4), "batched_agent_updates_per_s": w.n / (batch_ms / 1e3),
+            "looped_ms_per_step": round(loop_ms, 4), "looped_agent_updates_per_s": w.n / (loop_ms / 1e3),
+            "speedup_per_agent": round(loop_ms / batch_ms, 3), "timed_steps": steps, "looped_timed_steps": loop_steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +224,7 @@ def main():
     ap.add_argument("--workload", default="blobs_1m")
     ap.add_argument("--cpu-steps", type=int, default=10, help="CPU baseline sample steps (0 disables)")
     ap.add_argument("--no-phases", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and batched measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU: the step path has no multi-GPU form "
@@ -160,6 +278,12 @@ def main():
     total_b = sum(abytes[k] for k in ("hash_sort", "reorder_ranges", "neighbors_steer", "features"))
     step_gbs = total_b * w.n / 1e9 / (ms_per_step / 1e3)
 
+    ctx.close()
+    del fl
+    extras = {}
+    if not args.no_extras:
+        extras["adjoint"] = adjoint_ratio()
+        extras["batched"] = batched()
     cpu = cpu_baseline(w, args.cpu_steps) if args.cpu_steps > 0 else None
     line = {
         "metric": "agent-updates/s at 1M boids (% HBM peak); fwd+adjoint vs plain step time",
@@ -179,9 +303,9 @@ def main():
         "roofline": roofline,
         "phases": phases,
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(line), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

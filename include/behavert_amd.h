@@ -121,11 +121,16 @@ const char* brt_version(void);
  *   0 = every wave scans per lane).
  * BRT_OPT_XCD_REMAP: gather-heavy kernels map contiguous slices of the sorted
  *   agents to each XCD (one L2 per XCD on MI355X) (default 1).
+ * BRT_OPT_LDS_LISTS: the neighbour kernels stage each workgroup's lists in LDS
+ *   and write them out with coalesced 16-byte stores (1), or gather each list's
+ *   16-byte chunks in registers and store them as they fill (0), which leaves
+ *   LDS free and raises occupancy.
  * No counterpart in the reference (its launch shapes are fixed per kernel:
  * Proximity3DPlugIn.h:133-134, CommonResources.cpp:78-82). */
 #define BRT_OPT_FUSED_STEP 1
 #define BRT_OPT_UNIFORM_MAX_CELLS 2
 #define BRT_OPT_XCD_REMAP 3
+#define BRT_OPT_LDS_LISTS 4
 int brt_context_set_option(brt_context* ctx, int option, int value);
 int brt_context_get_option(const brt_context* ctx, int option, int* value);
 
@@ -208,6 +213,28 @@ int brt_osw_steer_integrate(brt_context* ctx, const brt_grid_params* grid,
                             float* smooth_accel_out, float* force_out,
                             void* stream);
 
+/* Adjoint (reverse mode) of brt_osw_steer_integrate with the sort and the lists
+ * held fixed as the forward step built them.  No counterpart in the reference.
+ * Inputs are the forward step's sorted state, hash and lists (exactly what the
+ * forward step read) and dL/d(new state) by agent ID:
+ *   adj_pos_out     float4: xyz = dL/dpos
+ *   adj_forward_out float4: xyz = dL/d(unit heading), w = dL/dspeed
+ *   adj_smooth_accel_out float4: xyz = dL/dsmooth_accel
+ * Outputs (overwritten) are dL/d(previous state) by agent ID in the same
+ * layout, and dL/d(separation, alignment, cohesion weight) ADDED to the three
+ * floats at weight_grad (device memory).  Neighbour contributions are summed
+ * with float atomics (the last bits depend on their order).
+ * Requires use_2d_projection == 0 and wander_weight == 0. */
+int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid,
+                         const brt_prox_params* prox, const brt_steer_params* steer,
+                         float dt, uint32_t n, const float* sorted_pos,
+                         const float* sorted_forward, const float* sorted_smooth_accel,
+                         const uint32_t* hash, const uint32_t* lists,
+                         const float* adj_pos_out, const float* adj_forward_out,
+                         const float* adj_smooth_accel_out, float* adj_pos_in,
+                         float* adj_forward_in, float* adj_smooth_accel_in,
+                         float* weight_grad, void* stream);
+
 /* ---------------------------------------------------------------- BehaviorClustering */
 
 /* Neighbour-averaged feature update (no GPU counterpart in the reference; the
@@ -261,6 +288,71 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid,
 #define BRT_NUM_PHASES 5
 int brt_phase_timing_enable(brt_context* ctx, int enable);
 int brt_phase_times(brt_context* ctx, double* ms_out, uint32_t* steps_out);
+
+/* ---------------------------------------------------------------- batches */
+
+/* Many independent flocks advanced together.  The reference runs one flock per
+ * process (BehaveRT_Flocking.cpp:177-196); a batch has no counterpart there.
+ * Each flock keeps its own grid (brt_grid_setup) and behaviour weights; the
+ * proximity and the other steering parameters are shared.  Flocks are
+ * concatenated in the order given: flock f owns agent IDs and sorted indices
+ * [offset_f, offset_f + n_f) and cell keys [cell_base_f, cell_base_f +
+ * num_cells_f) of the batch arrays, so that every flock's hash, sort, cell
+ * ranges, lists (minus offset_f) and new state equal those of brt_step run on
+ * that flock alone. */
+typedef struct brt_flock_desc {
+    brt_grid_params grid;
+    uint32_t n;
+    float weights[3]; /* separation, alignment, cohesion (override steer's [0]) */
+} brt_flock_desc;
+
+typedef struct brt_batch brt_batch;
+
+/* Tunables of the batched step.  prox_block: lanes per workgroup of the fused
+ * neighbour+steering kernel (64, 128 or 256; its LDS list tile is
+ * prox_block * brt_list_stride(K) * 4 bytes); uniform_max_cells: a wave whose
+ * agents share at most this many cells scans them cell by cell with
+ * wave-uniform loads (0..8); xcd_remap: 0/1; sort_items: keys per thread of
+ * the radix sort (8 or 16); lds_lists: the list tile lives in LDS (1) or the
+ * lists leave through per-lane register chunks (0, no LDS tile; see
+ * BRT_OPT_LDS_LISTS); use_graph: capture the step once into a hipGraph and
+ * replay it (1) or launch its kernels each call (0). */
+typedef struct brt_batch_config {
+    uint32_t prox_block;
+    uint32_t uniform_max_cells;
+    uint32_t xcd_remap;
+    uint32_t sort_items;
+    uint32_t lds_lists;
+    uint32_t use_graph;
+} brt_batch_config;
+
+int brt_batch_create(brt_context* ctx, const brt_flock_desc* flocks, uint32_t num_flocks,
+                     const brt_prox_params* prox, brt_batch** out);
+int brt_batch_destroy(brt_batch* batch);
+/* total agents and cells; offsets/cell_bases (num_flocks entries each) may be NULL */
+int brt_batch_layout(const brt_batch* batch, uint32_t* total_agents, uint32_t* total_cells,
+                     uint32_t* offsets, uint32_t* cell_bases);
+int brt_batch_set_config(brt_batch* batch, const brt_batch_config* cfg);
+int brt_batch_get_config(const brt_batch* batch, brt_batch_config* cfg);
+
+/* One step of every flock: hash + sort + reorder + ranges + fused neighbour
+ * lists and steering + integration (+ the feature update), over the batch
+ * arrays described by buf (buf->n = total agents; cell_start/cell_end hold
+ * total_cells entries; KNN lists are not supported).  With use_graph the
+ * sequence is captured on the first call and replayed while the buffers,
+ * parameters and configuration stay the same; the step number reaches the
+ * kernels through device memory, so one graph serves every step. */
+int brt_batch_step(brt_batch* batch, const brt_steer_params* steer, float feature_rate,
+                   float dt, uint32_t step, const brt_step_buffers* buf, void* stream);
+
+/* Times every configuration of the tunables above on the caller's buffers
+ * (their state is saved first and restored afterwards), keeps the fastest in
+ * the batch and returns it.  With cache_path != NULL the result is looked up
+ * in / appended to that text file under a key of (GPU architecture, flock
+ * sizes and grids, K), and a cached entry is used without timing. */
+int brt_batch_autotune(brt_batch* batch, const brt_steer_params* steer, float dt,
+                       const brt_step_buffers* buf, uint32_t steps_per_trial,
+                       const char* cache_path, brt_batch_config* chosen, void* stream);
 
 #ifdef __cplusplus
 }

@@ -59,7 +59,7 @@ def lib():
             "orc_neighbors": (None, [GP, PP, P, P, P, P, U, P]),
             "orc_neighbors_brute": (None, [GP, PP, P, P, U, P]),
             "orc_steer_f32": (None, [GP, PP, SP, F, U, U, P, P, P, P, P, P, P, P, P, P]),
-            "orc_steer_f64": (None, [GP, PP, SP, F, U, U, P, P, P, P, P, P, P, P, P, P, P]),
+            "orc_steer_f64": (None, [GP, PP, SP, F, U, U, P, P, P, P, P, P, P, P, P, P, P, P, P]),
             "orc_feature_update_f32": (None, [PP, F, U, P, P, P, P]),
             "orc_feature_update_f64": (None, [PP, F, U, P, P, P, P]),
             "orc_step_f32": (None, [GP, PP, SP, F, F, U, C.POINTER(StepBuffers), P]),
@@ -158,7 +158,8 @@ def neighbors_brute(grid, prox, spos, h) -> np.ndarray:
     return lists
 
 
-def _steer(fn, dtype, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None, cond=False):
+def _steer(fn, dtype, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None, cond=False,
+           pred=None):
     n = len(spos)
     spos = np.ascontiguousarray(spos, dtype)
     sfwd = np.ascontiguousarray(sfwd, dtype)
@@ -176,6 +177,9 @@ def _steer(fn, dtype, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wa
         k = np.zeros(n, dtype)
         args.append(_p(k))
         out["cond"] = k
+        pp = None if pred is None else np.ascontiguousarray(pred[0], np.float32)
+        pf = None if pred is None else np.ascontiguousarray(pred[1], np.float32)
+        args += [_p(pp), _p(pf)]
     fn(*args)
     return out
 
@@ -184,11 +188,13 @@ def steer_f32(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=Non
     return _steer(lib().orc_steer_f32, np.float32, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander)
 
 
-def steer_f64(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None):
+def steer_f64(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=None, pred=None):
     """FP64 evaluation; out["cond"][i] = max over the agent's behaviours of
-    sum|terms| / |sum of terms| (the condition number of its normalisations)."""
+    sum|terms| / |sum of terms| (the condition number of its normalisations).
+    pred = (sorted FP32 pos, sorted FP32 forward): take the discrete decisions
+    (radius/angle predicates, world box) from that state instead."""
     return _steer(lib().orc_steer_f64, np.float64, grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander,
-                  cond=True)
+                  cond=True, pred=pred)
 
 
 def feature_update_f32(prox, rate, h, lists, fin):
@@ -249,12 +255,24 @@ class HostFlock:
         return out
 
 
-def step_f64_fixed(grid, prox, steer, dt, step, state64, h, lists, wander64=None):
+def step_f64_fixed(grid, prox, steer, dt, step, state64, h, lists, wander64=None, pred=None):
     """One FP64 step from an FP64 state (agent-ID order) with the sort and the
-    neighbour lists fixed to (h, lists) (e.g. those the FP32 step produced)."""
+    neighbour lists fixed to (h, lists) (e.g. those the FP32 step produced);
+    pred = (sorted FP32 pos, sorted FP32 forward) also fixes the decisions."""
     ids = h[:, 1].astype(np.int64)
     spos = state64["pos"][ids]
     sfwd = state64["forward"][ids]
     ssa = state64["smooth_accel"][ids]
-    out = steer_f64(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander64)
+    out = steer_f64(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander64, pred=pred)
     return out
+
+
+def rollout_f64_fixed(grid, prox, steer, dt, records, state64, first_step=0):
+    """FP64 rollout along recorded FP32 steps: records[t] = (hash, lists,
+    sorted FP32 pos, sorted FP32 forward) of step t; sort, lists and discrete
+    decisions are taken from the record, the arithmetic is FP64."""
+    s = {k: np.array(state64[k], np.float64, copy=True) for k in ("pos", "forward", "smooth_accel")}
+    for t, (h, lists, spos32, sfwd32) in enumerate(records):
+        out = step_f64_fixed(grid, prox, steer, dt, first_step + t, s, h, lists, pred=(spos32, sfwd32))
+        s = {k: out[k] for k in ("pos", "forward", "smooth_accel")}
+    return s

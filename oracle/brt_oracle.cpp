@@ -291,6 +291,8 @@ struct SteerArrays {
     R* sa_out;
     R* force_out;    // sorted, optional
     R* cond_out;     // sorted, optional: max over behaviours of sum|terms| / |sum|
+    const float* pred_pos;  // sorted, optional: FP32 state the discrete decisions
+    const float* pred_fwd;  // (radius/angle predicates, world box) are taken from
 };
 
 template <typename R>
@@ -299,6 +301,17 @@ inline V3<R> ld3(const R* a, uint32_t i) { return v3<R>(a[4 * i], a[4 * i + 1], 
 template <typename R>
 void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& sp, R dt,
                  uint32_t step, const SteerArrays<R>& A) {
+    // state the discrete decisions are evaluated on: the agent's own (rounded
+    // to FP32), or a given FP32 state (finite-difference rollouts hold the
+    // forward's decisions fixed)
+    auto ppos = [&](uint32_t k) -> V3<R> {
+        return A.pred_pos ? v3<R>(R(A.pred_pos[4 * k]), R(A.pred_pos[4 * k + 1]), R(A.pred_pos[4 * k + 2]))
+                          : ld3(A.pos, k);
+    };
+    auto pfwd = [&](uint32_t k) -> V3<R> {
+        return A.pred_fwd ? v3<R>(R(A.pred_fwd[4 * k]), R(A.pred_fwd[4 * k + 1]), R(A.pred_fwd[4 * k + 2]))
+                          : ld3(A.fwd, k);
+    };
     const V3<R> myPos = ld3(A.pos, i);
     const V3<R> myFwd = ld3(A.fwd, i);
     R mySpeed = A.fwd[4 * i + 3];
@@ -325,7 +338,7 @@ void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& s
         for (uint32_t t = 0; t < count; ++t) {
             uint32_t j = L[1 + t];
             V3<R> other = ld3(A.pos, j);
-            if (inBoidNeighborhood(myPos, other, minR, sp.separation[1], sp.separation[2], myFwd)) {
+            if (inBoidNeighborhood(ppos(i), ppos(j), minR, sp.separation[1], sp.separation[2], pfwd(i))) {
                 V3<R> offset = other - myPos;
                 R d2 = dot(offset, offset);
                 s = s + offset / (-d2);
@@ -346,8 +359,7 @@ void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& s
         uint32_t nb = 0;
         for (uint32_t t = 0; t < count; ++t) {
             uint32_t j = L[1 + t];
-            V3<R> other = ld3(A.pos, j);
-            if (inBoidNeighborhood(myPos, other, minR, sp.alignment[1], sp.alignment[2], myFwd)) {
+            if (inBoidNeighborhood(ppos(i), ppos(j), minR, sp.alignment[1], sp.alignment[2], pfwd(i))) {
                 s = s + ld3(A.fwd, j);
                 mag = mag + length(ld3(A.fwd, j));
                 ++nb;
@@ -367,7 +379,7 @@ void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& s
         for (uint32_t t = 0; t < count; ++t) {
             uint32_t j = L[1 + t];
             V3<R> other = ld3(A.pos, j);
-            if (inBoidNeighborhood(myPos, other, minR, sp.cohesion[1], sp.cohesion[2], myFwd)) {
+            if (inBoidNeighborhood(ppos(i), ppos(j), minR, sp.cohesion[1], sp.cohesion[2], pfwd(i))) {
                 s = s + (other - myPos);
                 mag = mag + length(other - myPos);
                 ++nb;
@@ -412,7 +424,7 @@ void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& s
         // the box test is a discrete decision: FP32 on the FP32-rounded position
         const float cx = g.world_center[0], cy = g.world_center[1], cz = g.world_center[2];
         const float rx = g.world_radius[0], ry = g.world_radius[1], rz = g.world_radius[2];
-        const V3<float> q = f32v(myPos);
+        const V3<float> q = f32v(ppos(i));
         bool inside = (q.x > cx - rx) && (q.x < cx + rx) && (q.y > cy - ry) && (q.y < cy + ry) &&
                       (q.z > cz - rz) && (q.z < cz + rz);
         V3<R> s = inside ? v3<R>(0, 0, 0)
@@ -744,7 +756,7 @@ void orc_steer_f32(const brt_grid_params* g, const brt_prox_params* p, const brt
                    float* pos_out, float* fwd_out, float* sa_out, float* force_out) {
     set_threads();
     SteerArrays<float> A{spos, sfwd, ssa, hash, lists, listStride(p->max_neighbors),
-                         wander, pos_out, fwd_out, sa_out, force_out, nullptr};
+                         wander, pos_out, fwd_out, sa_out, force_out, nullptr, nullptr, nullptr};
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; ++i) steer_agent<float>((uint32_t)i, *g, *s, dt, step, A);
 }
@@ -752,10 +764,11 @@ void orc_steer_f32(const brt_grid_params* g, const brt_prox_params* p, const brt
 void orc_steer_f64(const brt_grid_params* g, const brt_prox_params* p, const brt_steer_params* s,
                    float dt, uint32_t step, uint32_t n, const double* spos, const double* sfwd,
                    const double* ssa, const uint32_t* hash, const uint32_t* lists, double* wander,
-                   double* pos_out, double* fwd_out, double* sa_out, double* force_out, double* cond_out) {
+                   double* pos_out, double* fwd_out, double* sa_out, double* force_out, double* cond_out,
+                   const float* pred_pos, const float* pred_fwd) {
     set_threads();
     SteerArrays<double> A{spos, sfwd, ssa, hash, lists, listStride(p->max_neighbors),
-                          wander, pos_out, fwd_out, sa_out, force_out, cond_out};
+                          wander, pos_out, fwd_out, sa_out, force_out, cond_out, pred_pos, pred_fwd};
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; ++i) steer_agent<double>((uint32_t)i, *g, *s, (double)dt, step, A);
 }

@@ -14,13 +14,25 @@ pytestmark = pytest.mark.gpu
 def gpu_lists(ctx, grid, prox, o):
     n = len(o["sorted_pos"])
     L = empty_u32(n, 4 * (prox.max_neighbors // 4 + 1))
+    L.fill_(0x5A5A5A5A)   # every word must be written
     ctx.neighbors(grid, prox, dev(o["sorted_pos"]), dev(o["hash"].view(np.int32)),
                   dev(o["cell_start"].view(np.int32)), dev(o["cell_end"].view(np.int32)), L)
     return u32(L)
 
 
+@pytest.fixture(params=[1, 0], ids=["lds_lists", "register_lists"])
+def list_mode(ctx, request):
+    """Runs the test with the lists staged in LDS and with the register-chunk
+    output; results must be identical."""
+    from behavert_amd.types import OPT_LDS_LISTS
+    old = ctx.get_option(OPT_LDS_LISTS)
+    ctx.set_option(OPT_LDS_LISTS, request.param)
+    yield request.param
+    ctx.set_option(OPT_LDS_LISTS, old)
+
+
 @pytest.mark.parametrize("make", [W.plain_4096, W.blobs_1m])
-def test_lists_bit_exact_on_workloads(ctx, orc, make):
+def test_lists_bit_exact_on_workloads(ctx, orc, make, list_mode):
     w = make()
     o = oracle_grid_outputs(orc, w.grid, w.prox, w.state)
     got = gpu_lists(ctx, w.grid, w.prox, o)
@@ -31,7 +43,7 @@ def test_lists_bit_exact_on_workloads(ctx, orc, make):
 
 
 @pytest.mark.parametrize("K", [1, 3, 4, 16, 63, 64, 252])
-def test_lists_bit_exact_across_caps(ctx, orc, K):
+def test_lists_bit_exact_across_caps(ctx, orc, K, list_mode):
     rng = np.random.default_rng(K)
     grid = GridParams.direct((0.0,) * 3, (4.0,) * 3, (12,) * 3)
     st = W.make_state(rng.uniform(0, 48, (20000, 3)).astype(np.float32),
@@ -41,7 +53,7 @@ def test_lists_bit_exact_across_caps(ctx, orc, K):
     assert first_mismatch(gpu_lists(ctx, grid, prox, o), o["lists"]) is None
 
 
-def test_edges_out_of_grid_nan_coincident_maxbodies(ctx, orc):
+def test_edges_out_of_grid_nan_coincident_maxbodies(ctx, orc, list_mode):
     rng = np.random.default_rng(5)
     for maxb in (0xFFFFFFFF, 3):
         grid = GridParams.direct((0.0,) * 3, (4.0,) * 3, (8,) * 3, max_bodies=maxb)
@@ -85,7 +97,7 @@ def test_radius_boundary_exact(ctx, orc):
 
 
 @pytest.mark.parametrize("uniform_max", [0, 1, 4, 64])
-def test_dense_with_clamped_boundaries_all_scan_modes(ctx, orc, uniform_max):
+def test_dense_with_clamped_boundaries_all_scan_modes(ctx, orc, uniform_max, list_mode):
     """Dense flock overflowing a small grid on every side: boundary cells hold
     clamped out-of-grid agents (the cell-box skip must not apply to them); the
     scan mode (uniform / per lane) must not change a single bit."""
@@ -105,7 +117,7 @@ def test_dense_with_clamped_boundaries_all_scan_modes(ctx, orc, uniform_max):
         ctx.set_option(OPT_UNIFORM_MAX_CELLS, old)
 
 
-def test_non_power_of_two_cells_offset_origin(ctx, orc):
+def test_non_power_of_two_cells_offset_origin(ctx, orc, list_mode):
     """Cell size 2.5 and an offset origin: cell boxes are not exact in float, so
     the skip margins are exercised."""
     rng = np.random.default_rng(123)

@@ -157,23 +157,27 @@ def test_step_fused_equals_decomposed_and_oracle_over_10_steps(ctx, orc):
 
 @pytest.mark.parametrize("make", [W.plain_4096, W.blobs_1m])
 def test_fused_step_equals_unfused_step(ctx, make):
-    """brt_step's fused neighbour+steering kernel and the two-kernel path give
-    the same bits (lists, hash, state) over 3 steps."""
-    from behavert_amd.types import OPT_FUSED_STEP
+    """brt_step's fused neighbour+steering kernel and the two-kernel path, each
+    with LDS-staged and register-chunk lists, give the same bits (lists, hash,
+    state) over 3 steps."""
+    from behavert_amd.types import OPT_FUSED_STEP, OPT_LDS_LISTS
     w = make()
     res = []
-    for fused in (1, 0):
+    for fused, lds in ((1, 1), (0, 1), (1, 0), (0, 0)):
         ctx.set_option(OPT_FUSED_STEP, fused)
+        ctx.set_option(OPT_LDS_LISTS, lds)
         try:
             fl = run_gpu_steps(ctx, w, 3)
             res.append((fl.state(), fl.grid_outputs()))
         finally:
             ctx.set_option(OPT_FUSED_STEP, 1)
-    (sa, ga), (sb, gb) = res
-    for k in ("pos", "forward", "smooth_accel"):
-        assert first_mismatch(sa[k].view(np.uint32), sb[k].view(np.uint32)) is None, k
-    for k in ("hash", "lists", "cell_start", "cell_end"):
-        assert first_mismatch(ga[k], gb[k]) is None, k
+            ctx.set_option(OPT_LDS_LISTS, 1)
+    sa, ga = res[0]
+    for sb, gb in res[1:]:
+        for k in ("pos", "forward", "smooth_accel"):
+            assert first_mismatch(sa[k].view(np.uint32), sb[k].view(np.uint32)) is None, k
+        for k in ("hash", "lists", "cell_start", "cell_end"):
+            assert first_mismatch(ga[k], gb[k]) is None, k
 
 
 def test_ten_step_trajectory_within_1e3_of_fp64(ctx, orc, record_property):
