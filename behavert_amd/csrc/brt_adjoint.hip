@@ -189,15 +189,21 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
                 const float Lbar = dot(o, inner);
                 const V3 gi = o * L;
                 fbar = fbar + gi * c;
-                const float cbar = dot(gi, f) + dot(gi, u) * (-c / pd);
                 const V3 ubar = gi * pd;
                 const V3 perpbar = (ubar - u * dot(u, ubar)) * (1.0f / pl);
                 Fbar = perpbar - f * dot(f, perpbar) + F * (Lbar / L);
                 fbar = fbar - perpbar * proj - F * dot(f, perpbar);
-                // c = 1 - 2 (s/maxAdj)^20
-                float r19 = 1.0f;
-                for (int k = 0; k < 19; ++k) r19 = r19 * range;
-                sbar = sbar + cbar * (-40.0f * r19 / maxAdj);
+                // c = 1 - 2x, x = (s/maxAdj)^20, and pd = sqrt(1 - c^2) = 2 r^10 sqrt(1 - x):
+                // dc/ds = -40 r^19 / maxAdj and dpd/ds = 20 c r^9 / (maxAdj sqrt(1 - x)), finite
+                // where the FP32 pd itself rounds to 0 (c rounds to 1 for small r)
+                float r9 = 1.0f;
+                for (int k = 0; k < 9; ++k) r9 = r9 * range;
+                const float r10 = r9 * range;
+                const float r19 = r9 * r10;
+                const float om = 1.0f - r10 * r10;
+                const float dc_ds = -40.0f * r19 / maxAdj;
+                const float dpd_ds = om > 0.0f ? 20.0f * c * r9 / (maxAdj * sqrtf(om)) : 0.0f;
+                sbar = sbar + dot(gi, f) * dc_ds + dot(gi, u) * dpd_ds;
             }
         }
         // seeks: d/d(p, f, s) of truncate(target - p, maxS) - f s

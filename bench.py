@@ -54,21 +54,34 @@ def algorithmic_bytes_per_agent(w) -> dict:
     return b
 
 
-PHASE_KERNELS = {"neighbors": "k_prox<false>", "steer_integrate": "k_steer", "hash_sort": "k_hash_hist+k_onesweep",
-                 "reorder_ranges": "k_reorder_ranges", "features": "k_features",
-                 "neighbors_steer": "k_prox<true>"}
+PHASE_KERNELS = {"neighbors": "k_prox<false,false,*>", "steer_integrate": "k_steer",
+                 "hash_sort": "k_prepare+k_hash_hist+k_onesweep", "reorder_ranges": "k_reorder_ranges",
+                 "features": "k_features", "neighbors_steer": "k_prox<true,false,*>"}
 
 
-def time_phases(ctx, run, first, steps, w, abytes, fused):
-    """Per-phase times (hipEvents on the step's stream) over `steps` steps."""
-    from behavert_amd.types import OPT_FUSED_STEP
+PROFILE_NAMES = {"neighbors_steer": "k_prox_fused", "neighbors": "k_prox_lists", "steer_integrate": "k_steer",
+                 "reorder_ranges": "k_reorder_ranges"}
 
-    ctx.set_option(OPT_FUSED_STEP, 1 if fused else 0)
-    ctx.phase_timing(True)
-    run(first, steps)
+
+def committed_traffic(phase: str, workload: str):
+    """HBM bytes per launch of the phase's kernel from the committed rocprofv3
+    counter summary (profiles/<kernel>_<workload>.md: FETCH_SIZE and WRITE_SIZE
+    scaled by the copy-kernel calibration), or None when there is none."""
+    name = PROFILE_NAMES.get(phase)
+    path = os.path.join(ROOT, "profiles", f"{name}_{workload}.md") if name else ""
+    if not name or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1].split()[0])
+    return None
+
+
+def phase_table(ctx, w, abytes, fused):
+    """Average per-phase times of the steps timed since phase timing was
+    enabled (HIP events recorded on the step's stream between the phases)."""
     ms, nsteps = ctx.phase_times()
-    ctx.phase_timing(False)
-    ctx.set_option(OPT_FUSED_STEP, 1)
     out = {}
     for k, v in ms.items():
         name = "neighbors_steer" if (fused and k == "neighbors") else k
@@ -166,6 +179,38 @@ def adjoint_ratio(steps=None, every=None) -> dict:
             "rollout_ratio": round((fwd_ms + bwd_ms) / fwd_ms, 3)}
 
 
+def full_stack(steps=20, warmup=3) -> dict:
+    """4,194,304 agents uniform in [0,640)^3, K 64: separation/alignment/cohesion +
+    seek toward 8 moving targets + wander + 4-D feature averaging (rate 0.1)."""
+    import torch
+
+    from behavert_amd import workloads as W
+    from behavert_amd.engine import Context, Flock
+
+    w = W.full_stack_4m()
+    ctx = Context(0)
+    fl = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander)
+
+    def run(first, count):
+        for i in range(first, first + count):
+            fl.steer.set_targets(w.targets_at(i))
+            fl.step(w.dt, i, w.feature_rate)
+
+    run(0, warmup)
+    torch.cuda.synchronize()
+    ctx.phase_timing(True)
+    t0 = time.perf_counter()
+    run(warmup, steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    abytes = algorithmic_bytes_per_agent(w)
+    ph = phase_table(ctx, w, abytes, fused=True)
+    ctx.phase_timing(False)
+    ctx.close()
+    return {**w.describe(), "ms_per_step": round(1e3 * el / steps, 4), "agent_updates_per_s": w.n * steps / el,
+            "phases": ph, "timed_steps": steps}
+
+
 def batched(steps=20, loop_steps=3) -> dict:
     """256 flocks (sizes log-uniform in [1024, 65536], 0.5 agents per unit^3, K 32,
     per-flock weights): the autotuned, graph-replayed batched step vs brt_step
@@ -225,6 +270,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10, help="CPU baseline sample steps (0 disables)")
     ap.add_argument("--no-phases", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and batched measurements")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with three 256 MiB device copies (the reference kernel for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU: the step path has no multi-GPU form "
@@ -246,8 +293,12 @@ def main():
                 fl.steer.set_targets(w.targets_at(i))
             fl.step(w.dt, i, w.feature_rate)
 
+    abytes = algorithmic_bytes_per_agent(w)
     run(0, args.warmup)
     torch.cuda.synchronize()
+    # the timed steps, with HIP events between the phases on the step's stream
+    if not args.no_phases:
+        ctx.phase_timing(True)
     t0 = time.perf_counter()
     run(args.warmup, args.steps)
     torch.cuda.synchronize()
@@ -255,19 +306,31 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     value = w.n * args.steps / elapsed
 
-    abytes = algorithmic_bytes_per_agent(w)
     phases = {}
     roofline = None
     if not args.no_phases:
+        phases["fused_step"] = phase_table(ctx, w, abytes, fused=True)
+        ctx.phase_timing(False)
+        # the two-kernel path (lists, then steering) on the following steps, for
+        # the grid + neighbour phases on their own
+        from behavert_amd.types import OPT_FUSED_STEP
+
+        ctx.set_option(OPT_FUSED_STEP, 0)
         first = args.warmup + args.steps
-        phases["fused_step"] = time_phases(ctx, run, first, args.steps, w, abytes, fused=True)
-        phases["unfused_step"] = time_phases(ctx, run, first + args.steps, args.steps, w, abytes, fused=False)
+        count = min(args.steps, 50)
+        ctx.phase_timing(True)
+        run(first, count)
+        phases["unfused_step"] = phase_table(ctx, w, abytes, fused=False)
+        phases["unfused_step"]["steps"] = [first, first + count]
+        ctx.phase_timing(False)
+        ctx.set_option(OPT_FUSED_STEP, 1)
         fz = phases["fused_step"]
-        dom = max((k for k in fz if abytes[k] > 0), key=lambda k: fz[k]["ms"])
+        dom = max((k for k in fz if isinstance(fz[k], dict) and abytes.get(k, 0) > 0), key=lambda k: fz[k]["ms"])
         roofline = {"bound": "hbm", "kernel": fz[dom]["kernel"], "phase": dom, "achieved": fz[dom]["GB/s"],
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fz[dom]["frac"], "traffic": None,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fz[dom]["frac"],
+                    "traffic": committed_traffic(dom, w.name),
                     "bytes_per_agent": round(abytes[dom], 2), "agents_per_launch": w.n,
-                    "avg_launch_ms": fz[dom]["ms"]}
+                    "avg_launch_ms": fz[dom]["ms"], "timed_steps": [args.warmup, args.warmup + args.steps]}
         uf = phases["unfused_step"]
         g_keys = ("hash_sort", "reorder_ranges", "neighbors")
         g_ms = sum(uf[k]["ms"] for k in g_keys)
@@ -280,10 +343,18 @@ def main():
 
     ctx.close()
     del fl
+    if args.copy_calibration:
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     extras = {}
     if not args.no_extras:
         extras["adjoint"] = adjoint_ratio()
         extras["batched"] = batched()
+        extras["full_stack"] = full_stack()
     cpu = cpu_baseline(w, args.cpu_steps) if args.cpu_steps > 0 else None
     line = {
         "metric": "agent-updates/s at 1M boids (% HBM peak); fwd+adjoint vs plain step time",
