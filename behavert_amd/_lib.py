@@ -10,7 +10,10 @@ import os
 
 from .types import BatchConfig, FlockDesc, GridParams, ProxParams, SteerParams, StepBuffers
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbehavert_amd.so")
+# BRT_LIB names another build of the same library (e.g. one compiled with
+# different launch constants); the default is the in-tree build.
+LIB_PATH = os.environ.get("BRT_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                      "libbehavert_amd.so")
 
 _P = C.c_void_p
 _U32 = C.c_uint32

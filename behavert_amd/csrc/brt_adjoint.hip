@@ -84,11 +84,24 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
         SteerAccum acc;
         acc.init();
         if (behaviours)
-            for (uint32_t t = 0; t < count; ++t) {
-                const uint32_t j = list_entry(row, t);
-                if (j >= a.n) continue;
-                const V3 off = xyz(a.spos[j]) - p;
-                acc.add(sp, weights_of(sp), f, off, dot(off, off), xyz(a.sfwd[j]));
+            for (uint32_t t = 0; t < count; t += 4) {
+                // four neighbours' gathers in flight
+                uint32_t jj[4];
+                float4 q[4], fq[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t j = (t + u < count) ? list_entry(row, t + u) : a.n;
+                    jj[u] = j;
+                    const uint32_t js = j < a.n ? j : i;
+                    q[u] = a.spos[js];
+                    fq[u] = a.sfwd[js];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (jj[u] >= a.n) continue;
+                    const V3 off = xyz(q[u]) - p;
+                    acc.add(sp, weights_of(sp), f, off, dot(off, off), xyz(fq[u]));
+                }
             }
         V3 xs = v3(0.f, 0.f, 0.f), xa = xs, xc = xs, us = xs, ua = xs, uc = xs;
         V3 F = v3(0.f, 0.f, 0.f);
@@ -236,37 +249,50 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
             Cbar = normalize_bwd(xc, Fbar * sp.cohesion[0]) * (1.0f / (float)acc.nCoh);
         }
         if (behaviours)
-            for (uint32_t t = 0; t < count; ++t) {
-                const uint32_t j = list_entry(row, t);
-                if (j >= a.n) continue;
-                const uint32_t jid = a.hash[j].y;
-                if (jid >= a.n) continue;
-                const V3 off = xyz(a.spos[j]) - p;
-                const float d2 = dot(off, off);
-                const float dist = sqrtf(d2);
-                const float cosA = dot(f, off * (1.0f / dist));
-                const float minR = sp.sensing_min_radius;
-                V3 offbar = v3(0.f, 0.f, 0.f);
-                bool any = false;
-                if (acc.nSep > 0 && sp.separation[0] != 0.0f &&
-                    inBoid(dist, cosA, minR, sp.separation[1], sp.separation[2])) {
-                    // d/d(off) of -off/|off|^2
-                    const float inv = 1.0f / d2;
-                    offbar = offbar - Sbar * inv + off * (2.0f * dot(off, Sbar) * inv * inv);
-                    any = true;
+            for (uint32_t t = 0; t < count; t += 4) {
+                uint32_t jj[4], jid4[4];
+                float4 q[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t j = (t + u < count) ? list_entry(row, t + u) : a.n;
+                    jj[u] = j;
+                    const uint32_t js = j < a.n ? j : i;
+                    q[u] = a.spos[js];
+                    jid4[u] = a.hash[js].y;
                 }
-                if (acc.nCoh > 0 && sp.cohesion[0] != 0.0f &&
-                    inBoid(dist, cosA, minR, sp.cohesion[1], sp.cohesion[2])) {
-                    offbar = offbar + Cbar;
-                    any = true;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t j = jj[u];
+                    if (j >= a.n) continue;
+                    const uint32_t jid = jid4[u];
+                    if (jid >= a.n) continue;
+                    const V3 off = xyz(q[u]) - p;
+                    const float d2 = dot(off, off);
+                    const float dist = sqrtf(d2);
+                    const float cosA = dot(f, off * (1.0f / dist));
+                    const float minR = sp.sensing_min_radius;
+                    V3 offbar = v3(0.f, 0.f, 0.f);
+                    bool any = false;
+                    if (acc.nSep > 0 && sp.separation[0] != 0.0f &&
+                        inBoid(dist, cosA, minR, sp.separation[1], sp.separation[2])) {
+                        // d/d(off) of -off/|off|^2
+                        const float inv = 1.0f / d2;
+                        offbar = offbar - Sbar * inv + off * (2.0f * dot(off, Sbar) * inv * inv);
+                        any = true;
+                    }
+                    if (acc.nCoh > 0 && sp.cohesion[0] != 0.0f &&
+                        inBoid(dist, cosA, minR, sp.cohesion[1], sp.cohesion[2])) {
+                        offbar = offbar + Cbar;
+                        any = true;
+                    }
+                    if (any) {
+                        atomic_add3(&a.gpos_in[jid], offbar);
+                        pbar = pbar - offbar;
+                    }
+                    if (acc.nAli > 0 && sp.alignment[0] != 0.0f &&
+                        inBoid(dist, cosA, minR, sp.alignment[1], sp.alignment[2]))
+                        atomic_add3(&a.gfwd_in[jid], Abar);
                 }
-                if (any) {
-                    atomic_add3(&a.gpos_in[jid], offbar);
-                    pbar = pbar - offbar;
-                }
-                if (acc.nAli > 0 && sp.alignment[0] != 0.0f &&
-                    inBoid(dist, cosA, minR, sp.alignment[1], sp.alignment[2]))
-                    atomic_add3(&a.gfwd_in[jid], Abar);
             }
         atomic_add3(&a.gpos_in[id], pbar);
         atomic_add3(&a.gfwd_in[id], fbar);
@@ -301,14 +327,32 @@ __global__ void __launch_bounds__(256) k_zero3(float4* __restrict__ a, float4* _
     }
 }
 
-// Sums the per-workgroup partials in workgroup order (one lane per weight) and
-// adds the result to weight_grad.
-__global__ void k_reduce_wgrad(const float* __restrict__ partials, uint32_t blocks, float* __restrict__ weight_grad) {
-    const uint32_t k = threadIdx.x;
-    if (k >= 3) return;
-    float v = 0.0f;
-    for (uint32_t b = 0; b < blocks; ++b) v += partials[(size_t)b * 3 + k];
-    weight_grad[k] = weight_grad[k] + v;
+// Sums the per-workgroup partials in a fixed order (strided per-thread sums,
+// then a fixed LDS tree) and adds the result to weight_grad; the same inputs
+// always give the same bits.
+__global__ void __launch_bounds__(256) k_reduce_wgrad(const float* __restrict__ partials, uint32_t blocks,
+                                                      float* __restrict__ weight_grad) {
+    __shared__ float s_sum[3][256];
+    const uint32_t tid = threadIdx.x;
+    float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+    for (uint32_t b = tid; b < blocks; b += 256u) {
+        v0 += partials[(size_t)b * 3 + 0];
+        v1 += partials[(size_t)b * 3 + 1];
+        v2 += partials[(size_t)b * 3 + 2];
+    }
+    s_sum[0][tid] = v0;
+    s_sum[1][tid] = v1;
+    s_sum[2][tid] = v2;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (tid < o) {
+            s_sum[0][tid] += s_sum[0][tid + o];
+            s_sum[1][tid] += s_sum[1][tid + o];
+            s_sum[2][tid] += s_sum[2][tid + o];
+        }
+        __syncthreads();
+    }
+    if (tid < 3) weight_grad[tid] = weight_grad[tid] + s_sum[tid][0];
 }
 
 }  // namespace
@@ -341,7 +385,7 @@ hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_
     a.partials = partials;
     a.xcd_remap = xcd_remap;
     hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_reduce_wgrad, dim3(1), dim3(64), 0, s, partials, blocks, weight_grad);
+    hipLaunchKernelGGL(k_reduce_wgrad, dim3(1), dim3(256), 0, s, partials, blocks, weight_grad);
     return hipGetLastError();
 }
 

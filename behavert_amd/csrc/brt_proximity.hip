@@ -34,9 +34,17 @@
 #include "brt_internal.hpp"
 #include "brt_steer_common.hpp"
 
+// Candidates tested per round of the scan loops (their loads are in flight
+// together).
+#ifndef BRT_PROX_GROUP
+#define BRT_PROX_GROUP 4
+#endif
+
 namespace brt {
 
 namespace {
+
+constexpr int kGroup = BRT_PROX_GROUP;
 
 // Loads through the constant address space: with a wave-uniform index these
 // become scalar-cache loads (s_load_dwordx4).  The arrays are not written by
@@ -239,12 +247,12 @@ __device__ __forceinline__ void scan_uniform(const ProxArgs& a, const Seg& sg, L
                     const bool need = (L.cnt < a.K) & !(bd > a.Tskip);
                     if (__ballot(need) == 0) continue;
                 }
-                for (uint32_t j = s; j < lim; j += 4) {
-                    float4 q[4];
+                for (uint32_t j = s; j < lim; j += kGroup) {
+                    float4 q[kGroup];
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) q[t] = cload(cpos, (j + t < lim) ? j + t : lim - 1);
+                    for (int t = 0; t < kGroup; ++t) q[t] = cload(cpos, (j + t < lim) ? j + t : lim - 1);
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) test_candidate<LDSL>(a, L, j + t, lim, q[t]);
+                    for (int t = 0; t < kGroup; ++t) test_candidate<LDSL>(a, L, j + t, lim, q[t]);
                     if (__ballot(L.cnt < a.K) == 0) return;
                 }
             }
@@ -282,12 +290,12 @@ __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, La
             const uint32_t e = en[k];
             uint32_t lim = (e - s) > maxBodies ? s + maxBodies : e;
             lim = lim < a.n ? lim : a.n;
-            for (uint32_t j = s; j < lim; j += 4) {
-                float4 q[4];
+            for (uint32_t j = s; j < lim; j += kGroup) {
+                float4 q[kGroup];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) q[t] = a.spos[(j + t < lim) ? j + t : lim - 1];
+                for (int t = 0; t < kGroup; ++t) q[t] = a.spos[(j + t < lim) ? j + t : lim - 1];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) test_candidate<LDSL>(a, L, j + t, lim, q[t]);
+                for (int t = 0; t < kGroup; ++t) test_candidate<LDSL>(a, L, j + t, lim, q[t]);
                 if (L.cnt >= a.K) return;
             }
         }
