@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .types import BatchConfig, FlockDesc, GridParams, ProxParams, SteerParams, StepBuffers
+from .types import BatchConfig, BcParams, FlockDesc, GridParams, ProxParams, SteerParams, StepBuffers
 
 # BRT_LIB names another build of the same library (e.g. one compiled with
 # different launch constants); the default is the in-tree build.
@@ -31,6 +31,7 @@ SIGNATURES = {
     "brt_version": (C.c_char_p, []),
     "brt_context_set_option": (_I, [_P, _I, _I]),
     "brt_context_get_option": (_I, [_P, _I, C.POINTER(_I)]),
+    "brt_context_step_shape": (_I, [_P, _PP, C.POINTER(_I), C.POINTER(_I)]),
     "brt_grid_setup": (_I, [C.POINTER(_F), C.POINTER(_F), C.POINTER(_F), _U32, _GP]),
     "brt_list_stride": (_U32, [_U32]),
     "brt_radius_threshold": (_F, [_F]),
@@ -44,6 +45,12 @@ SIGNATURES = {
     "brt_osw_steer_integrate": (_I, [_P, _GP, _PP, _SP, _F, _U32, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                      _P, _P]),
     "brt_bc_feature_update": (_I, [_P, _PP, _F, _U32, _P, _P, _P, _P, _P]),
+    "brt_osw_steer_forces": (_I, [_P, _PP, _SP, _F, _U32, _U32, _P, _P, _P, _P, _P, _P, _P]),
+    "brt_osw_apply_steering_force": (_I, [_P, _GP, _SP, _F, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "brt_bc_neigh_similarities": (_I, [_P, _PP, C.POINTER(BcParams), _U32, _P, _P, _P, _P, _P, _P, _P]),
+    "brt_bc_steer_for_clustering": (_I, [_P, _PP, _SP, C.POINTER(BcParams), _U32, _P, _P, _P, _P, _P, _P, _P]),
+    "brt_bc_cluster_membership": (_I, [_P, _PP, _U32, _U32, _P, _P, _P, _P, _P, _P, _P]),
+    "brt_bc_cluster_hit_rate": (_I, [_P, _PP, _F, _U32, _P, _P, _P, _P, _P, _P]),
     "brt_osw_adjoint_step": (_I, [_P, _GP, _PP, _SP, _F, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "brt_step": (_I, [_P, _GP, _PP, _SP, _F, _F, _U32, C.POINTER(StepBuffers), _P]),
     "brt_phase_timing_enable": (_I, [_P, _I]),

@@ -60,20 +60,40 @@ class Rollout:
         self.g = [(f4(n), f4(n), f4(n)), (f4(n), f4(n), f4(n))]
         self.wgrad = torch.zeros(3, dtype=torch.float32, device=dev)
         self.checkpoints = []
+        # argument blocks built once: the per-step calls only pass them on
+        self._bufs = self._buffers(self.spos, self.sfwd, self.ssa, self.hash, self.lists)
+        self._rec_bufs = [self._buffers(self.r_spos[k], self.r_sfwd[k], self.r_ssa[k], self.r_hash[k],
+                                        self.r_lists[k]) for k in range(self.K)]
+        self._adj_args = {}
 
     def set_weights(self, w):
         w = [float(x) for x in w]
         self.steer.separation[0], self.steer.alignment[0], self.steer.cohesion[0] = w
 
-    def _step(self, t, spos, sfwd, ssa, hash_, lists):
+    def _buffers(self, spos, sfwd, ssa, hash_, lists) -> StepBuffers:
         b = StepBuffers()
         b.n = self.n
         b.pos, b.forward, b.smooth_accel = _p(self.pos), _p(self.fwd), _p(self.sa)
         b.sorted_pos, b.sorted_forward, b.sorted_smooth_accel = _p(spos), _p(sfwd), _p(ssa)
         b.hash, b.lists = _p(hash_), _p(lists)
         b.cell_start, b.cell_end = _p(self.cell_start), _p(self.cell_end)
+        return b
+
+    def _step(self, t, bufs: StepBuffers):
         self.ctx.check(lib().brt_step(self.ctx.handle, C.byref(self.grid), C.byref(self.prox), C.byref(self.steer),
-                                      0.0, self.dt, int(t), C.byref(b), current_stream()))
+                                      0.0, self.dt, int(t), C.byref(bufs), current_stream()))
+
+    def _adjoint(self, k, cur):
+        key = (k, cur)
+        args = self._adj_args.get(key)
+        if args is None:
+            go, gi = self.g[cur], self.g[1 - cur]
+            args = (self.ctx.handle, C.byref(self.grid), C.byref(self.prox), C.byref(self.steer), self.dt, self.n,
+                    _p(self.r_spos[k]), _p(self.r_sfwd[k]), _p(self.r_ssa[k]), _p(self.r_hash[k]),
+                    _p(self.r_lists[k]), _p(go[0]), _p(go[1]), _p(go[2]), _p(gi[0]), _p(gi[1]), _p(gi[2]),
+                    _p(self.wgrad))
+            self._adj_args[key] = args
+        self.ctx.check(lib().brt_osw_adjoint_step(*args, current_stream()))
 
     def forward(self, weights, pos0, fwd0, sa0):
         """Runs the rollout; keeps checkpoints; returns the final (pos, forward, smooth_accel)."""
@@ -85,12 +105,11 @@ class Rollout:
         for t in range(self.steps):
             if t % self.K == 0:
                 self.checkpoints.append((t, self.pos.clone(), self.fwd.clone(), self.sa.clone()))
-            self._step(t, self.spos, self.sfwd, self.ssa, self.hash, self.lists)
+            self._step(t, self._bufs)
         return self.pos.clone(), self.fwd.clone(), self.sa.clone()
 
     def backward(self, gpos, gfwd, gsa):
         """dL/d(final state) -> dL/d(weights, pos0, forward0, smooth_accel0)."""
-        L = lib()
         cur = 0
         self.g[cur][0].copy_(gpos)
         self.g[cur][1].copy_(gfwd)
@@ -102,16 +121,9 @@ class Rollout:
             self.fwd.copy_(f)
             self.sa.copy_(a)
             for t in range(t0, t1):
-                k = t - t0
-                self._step(t, self.r_spos[k], self.r_sfwd[k], self.r_ssa[k], self.r_hash[k], self.r_lists[k])
+                self._step(t, self._rec_bufs[t - t0])
             for t in range(t1 - 1, t0 - 1, -1):
-                k = t - t0
-                go, gi = self.g[cur], self.g[1 - cur]
-                self.ctx.check(L.brt_osw_adjoint_step(
-                    self.ctx.handle, C.byref(self.grid), C.byref(self.prox), C.byref(self.steer), self.dt, self.n,
-                    _p(self.r_spos[k]), _p(self.r_sfwd[k]), _p(self.r_ssa[k]), _p(self.r_hash[k]),
-                    _p(self.r_lists[k]), _p(go[0]), _p(go[1]), _p(go[2]), _p(gi[0]), _p(gi[1]), _p(gi[2]),
-                    _p(self.wgrad), current_stream()))
+                self._adjoint(t - t0, cur)
                 cur = 1 - cur
         g = self.g[cur]
         return self.wgrad.clone(), g[0].clone(), g[1].clone(), g[2].clone()

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbehavert_amd.so")
-SOURCES = ["brt_envgrid.hip", "brt_proximity.hip", "brt_opensteer.hip", "brt_batch.hip",
+SOURCES = ["brt_envgrid.hip", "brt_proximity.hip", "brt_opensteer.hip", "brt_clustering.hip", "brt_batch.hip",
            "brt_adjoint.hip", "brt_capi.hip"]
 ARCH = os.environ.get("BRT_OFFLOAD_ARCH", "gfx950")
 
@@ -42,19 +42,23 @@ def hipcc() -> str:
 
 EXAMPLE_SRC = os.path.join(HERE, "examples", "flocking_demo.cpp")
 EXAMPLE_BIN = os.path.join(HERE, "examples", "flocking_demo")
+CLUSTERING_SRC = os.path.join(HERE, "examples", "clustering_demo.cpp")
+CLUSTERING_BIN = os.path.join(HERE, "examples", "clustering_demo")
 
 
 def build_example(verbose: bool = False) -> str:
-    """The plug-in layer's BehaveRT_Flocking-style driver, linked to the library."""
-    cmd = [hipcc(), "-std=c++17", "-O2", "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
-           "-packed-fp32-ops", "-Wall", "-Wno-unused-command-line-argument", "-o", EXAMPLE_BIN, EXAMPLE_SRC,
-           "-L", HERE, "-lbehavert_amd", "-Wl,-rpath,$ORIGIN/.."]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    proc = subprocess.run(cmd, capture_output=True, text=True)
-    if proc.returncode != 0:
-        sys.stderr.write(proc.stdout + proc.stderr)
-        raise RuntimeError(f"building the plug-in example failed with exit code {proc.returncode}")
+    """The plug-in layer's BehaveRT_Flocking- and BehaveRT_Clustering-style
+    drivers, linked to the library."""
+    for src, out in ((EXAMPLE_SRC, EXAMPLE_BIN), (CLUSTERING_SRC, CLUSTERING_BIN)):
+        cmd = [hipcc(), "-std=c++17", "-O2", "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
+               "-packed-fp32-ops", "-Wall", "-Wno-unused-command-line-argument", "-o", out, src,
+               "-L", HERE, "-lbehavert_amd", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        proc = subprocess.run(cmd, capture_output=True, text=True)
+        if proc.returncode != 0:
+            sys.stderr.write(proc.stdout + proc.stderr)
+            raise RuntimeError(f"building {os.path.basename(src)} failed with exit code {proc.returncode}")
     return EXAMPLE_BIN
 
 

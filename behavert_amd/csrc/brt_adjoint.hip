@@ -60,11 +60,41 @@ struct AdjArgs {
     float4* gpos_in;
     float4* gfwd_in;
     float4* gsa_in;
-    float* partials;  // per workgroup: 3 floats
+    float* partials;         // per workgroup: 3 floats
+    uint32_t* done;          // workgroups finished (reset to 0 by the last one)
+    float* weight_grad;
     uint32_t xcd_remap;
 };
 
 __device__ __forceinline__ uint32_t list_entry(const uint32_t* row, uint32_t t) { return row[1 + t]; }
+
+// Sums the per-workgroup partials in a fixed order (strided per-thread sums,
+// then a fixed LDS tree) and adds the result to weight_grad; the same inputs
+// always give the same bits.
+// (256 threads)
+__device__ void reduce_partials(const float* __restrict__ partials, uint32_t blocks, float* __restrict__ weight_grad) {
+    __shared__ float s_sum[3][256];
+    const uint32_t tid = threadIdx.x;
+    float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+    for (uint32_t b = tid; b < blocks; b += 256u) {
+        v0 += partials[(size_t)b * 3 + 0];
+        v1 += partials[(size_t)b * 3 + 1];
+        v2 += partials[(size_t)b * 3 + 2];
+    }
+    s_sum[0][tid] = v0;
+    s_sum[1][tid] = v1;
+    s_sum[2][tid] = v2;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (tid < o) {
+            s_sum[0][tid] += s_sum[0][tid + o];
+            s_sum[1][tid] += s_sum[1][tid + o];
+            s_sum[2][tid] += s_sum[2][tid + o];
+        }
+        __syncthreads();
+    }
+    if (tid < 3) weight_grad[tid] = weight_grad[tid] + s_sum[tid][0];
+}
 
 __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
     __shared__ float s_red[3][4];
@@ -314,6 +344,18 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
         for (uint32_t w = 0; w < nw; ++w) v += s_red[threadIdx.x][w];
         a.partials[(size_t)blockIdx.x * 3 + threadIdx.x] = v;
     }
+    // the last workgroup to finish sums every workgroup's partials in a fixed
+    // order (no wait: it only runs once all others have published)
+    __shared__ bool s_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(a.done, 1u) == gridDim.x - 1u;
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        reduce_partials(a.partials, gridDim.x, a.weight_grad);
+        if (threadIdx.x == 0) __hip_atomic_store(a.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_zero3(float4* __restrict__ a, float4* __restrict__ b,
@@ -327,41 +369,14 @@ __global__ void __launch_bounds__(256) k_zero3(float4* __restrict__ a, float4* _
     }
 }
 
-// Sums the per-workgroup partials in a fixed order (strided per-thread sums,
-// then a fixed LDS tree) and adds the result to weight_grad; the same inputs
-// always give the same bits.
-__global__ void __launch_bounds__(256) k_reduce_wgrad(const float* __restrict__ partials, uint32_t blocks,
-                                                      float* __restrict__ weight_grad) {
-    __shared__ float s_sum[3][256];
-    const uint32_t tid = threadIdx.x;
-    float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
-    for (uint32_t b = tid; b < blocks; b += 256u) {
-        v0 += partials[(size_t)b * 3 + 0];
-        v1 += partials[(size_t)b * 3 + 1];
-        v2 += partials[(size_t)b * 3 + 2];
-    }
-    s_sum[0][tid] = v0;
-    s_sum[1][tid] = v1;
-    s_sum[2][tid] = v2;
-    __syncthreads();
-    for (uint32_t o = 128; o > 0; o >>= 1) {
-        if (tid < o) {
-            s_sum[0][tid] += s_sum[0][tid + o];
-            s_sum[1][tid] += s_sum[1][tid + o];
-            s_sum[2][tid] += s_sum[2][tid + o];
-        }
-        __syncthreads();
-    }
-    if (tid < 3) weight_grad[tid] = weight_grad[tid] + s_sum[tid][0];
-}
-
 }  // namespace
 
 hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t n,
                           const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
                           const uint32_t* lists, const float4* gpos_out, const float4* gfwd_out,
                           const float4* gsa_out, float4* gpos_in, float4* gfwd_in, float4* gsa_in,
-                          float* partials, float* weight_grad, uint32_t xcd_remap, hipStream_t s) {
+                          float* partials, uint32_t* done, float* weight_grad, uint32_t xcd_remap,
+                          hipStream_t s) {
     const uint32_t blocks = (n + 255) / 256;
     uint32_t zgrid = blocks < 2048 ? blocks : 2048;
     hipLaunchKernelGGL(k_zero3, dim3(zgrid), dim3(256), 0, s, gpos_in, gfwd_in, gsa_in, n);
@@ -383,9 +398,10 @@ hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_
     a.gfwd_in = gfwd_in;
     a.gsa_in = gsa_in;
     a.partials = partials;
+    a.done = done;
+    a.weight_grad = weight_grad;
     a.xcd_remap = xcd_remap;
     hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_reduce_wgrad, dim3(1), dim3(256), 0, s, partials, blocks, weight_grad);
     return hipGetLastError();
 }
 

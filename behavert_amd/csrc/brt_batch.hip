@@ -254,7 +254,7 @@ int brt_batch_create(brt_context* ctx, const brt_flock_desc* flocks, uint32_t nu
         total_cells += flocks[f].grid.num_cells;
         waves += (flocks[f].n + 63u) / 64u;
     }
-    BRT_REQUIRE(total_n < (1ull << 30), "a batch holds fewer than 2^30 agents");
+    BRT_REQUIRE(total_n < (1ull << 28), "a batch holds fewer than 2^28 agents (32-bit byte offsets)");
     BRT_REQUIRE(total_cells < (1ull << 31), "a batch holds fewer than 2^31 cells");
     if ((rc = use_device(ctx))) return rc;
 

@@ -66,6 +66,11 @@ int check_n(brt_context* ctx, uint32_t n) {
     return BRT_OK;
 }
 
+int check_n_lists(brt_context* ctx, uint32_t n) {
+    BRT_REQUIRE(n < (1u << 28), "n = %u: the neighbour search handles fewer than 2^28 agents per launch", n);
+    return BRT_OK;
+}
+
 int check_prox(brt_context* ctx, const brt_prox_params* p) {
     BRT_REQUIRE(p != nullptr, "proximity params are NULL");
     BRT_REQUIRE(p->max_neighbors >= 1 && p->max_neighbors <= BRT_MAX_NEIGHBORS,
@@ -80,16 +85,21 @@ int check_steer(brt_context* ctx, const brt_steer_params* s) {
     return BRT_OK;
 }
 
-int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan) {
-    size_t need_tmp = (size_t)n * 8u;
+int ensure_tmp(brt_context* ctx, size_t need_tmp) {
     if (need_tmp > ctx->tmp_bytes) {
-        if (ctx->tmp) BRT_HIP(hipFree(ctx->tmp), "hipFree(sort tmp)");
+        if (ctx->tmp) BRT_HIP(hipFree(ctx->tmp), "hipFree(tmp)");
         ctx->tmp = nullptr;
         ctx->tmp_bytes = 0;
         size_t bytes = need_tmp + need_tmp / 4 + 4096;
-        BRT_HIP(hipMalloc(&ctx->tmp, bytes), "hipMalloc(sort tmp)");
+        BRT_HIP(hipMalloc(&ctx->tmp, bytes), "hipMalloc(tmp)");
         ctx->tmp_bytes = bytes;
     }
+    return BRT_OK;
+}
+
+int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan) {
+    int rc = ensure_tmp(ctx, (size_t)n * 8u);
+    if (rc) return rc;
     size_t need_meta = brt::sort_meta_words(plan) * 4u;
     if (need_meta > ctx->meta_bytes) {
         if (ctx->meta) BRT_HIP(hipFree(ctx->meta), "hipFree(sort meta)");
@@ -177,7 +187,8 @@ int brt_context_set_option(brt_context* ctx, int option, int value) {
     if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
     switch (option) {
         case BRT_OPT_FUSED_STEP:
-            ctx->fused_step = value != 0;
+            BRT_REQUIRE(value >= -1, "fused_step must be -1 (automatic), 0 or 1");
+            ctx->fused_step = value < 0 ? -1 : (value != 0);
             return BRT_OK;
         case BRT_OPT_UNIFORM_MAX_CELLS:
             BRT_REQUIRE(value >= 0 && value <= 64, "uniform_max_cells must be in 0..64");
@@ -187,7 +198,8 @@ int brt_context_set_option(brt_context* ctx, int option, int value) {
             ctx->xcd_remap = value != 0;
             return BRT_OK;
         case BRT_OPT_LDS_LISTS:
-            ctx->lds_lists = value != 0;
+            BRT_REQUIRE(value >= -1, "lds_lists must be -1 (automatic), 0 or 1");
+            ctx->lds_lists = value < 0 ? -1 : (value != 0);
             return BRT_OK;
         default:
             return set_err(ctx, BRT_ERR_INVALID_ARGUMENT, "unknown option %d", option);
@@ -207,11 +219,18 @@ int brt_context_get_option(const brt_context* ctx, int option, int* value) {
             *value = (int)ctx->xcd_remap;
             return BRT_OK;
         case BRT_OPT_LDS_LISTS:
-            *value = (int)ctx->lds_lists;
+            *value = ctx->lds_lists;
             return BRT_OK;
         default:
             return BRT_ERR_INVALID_ARGUMENT;
     }
+}
+
+int brt_context_step_shape(const brt_context* ctx, const brt_prox_params* prox, int* fused, int* lds_lists) {
+    if (!ctx || !prox || !fused || !lds_lists) return BRT_ERR_INVALID_ARGUMENT;
+    *fused = resolve_fused(ctx, prox) && !prox->use_knn ? 1 : 0;
+    *lds_lists = (int)resolve_lds_lists(ctx, prox);
+    return BRT_OK;
 }
 
 int brt_grid_setup(const float world_center[3], const float world_radius[3], const float cell_size[3],
@@ -367,6 +386,7 @@ int brt_prox_neighbors(brt_context* ctx, const brt_grid_params* grid, const brt_
     if (rc) return rc;
     if ((rc = check_prox(ctx, prox))) return rc;
     if ((rc = check_n(ctx, n))) return rc;
+    if ((rc = check_n_lists(ctx, n))) return rc;
     if (n == 0) return BRT_OK;
     BRT_REQUIRE(sorted_pos && hash && cell_start && cell_end && lists, "NULL array");
     BRT_REQUIRE(aligned16(sorted_pos) && aligned16(lists), "sorted_pos and lists must be 16-byte aligned");
@@ -374,7 +394,7 @@ int brt_prox_neighbors(brt_context* ctx, const brt_grid_params* grid, const brt_
     BRT_HIP(brt::launch_neighbors(*grid, *prox, brt_radius_threshold(prox->search_radius),
                                   reinterpret_cast<const float4*>(sorted_pos), reinterpret_cast<const uint2*>(hash),
                                   cell_start, cell_end, n, lists, ctx->uniform_max_cells, ctx->xcd_remap,
-                                  ctx->lds_lists, (hipStream_t)stream),
+                                  resolve_lds_lists(ctx, prox), (hipStream_t)stream),
             "k_neighbors");
     return BRT_OK;
 }
@@ -439,14 +459,17 @@ int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid, const br
                     adj_smooth_accel_in != adj_smooth_accel_out,
                 "adjoint input and output arrays must differ");
     if ((rc = use_device(ctx))) return rc;
-    const size_t need = ((size_t)(n + 255) / 256) * 3 * sizeof(float);
+    // per-workgroup weight-gradient partials, then one counter word at the end
+    const size_t need = ((size_t)(n + 255) / 256) * 3 * sizeof(float) + 256;
     if (need > ctx->partials_bytes) {
         if (ctx->partials) BRT_HIP(hipFree(ctx->partials), "hipFree(partials)");
         ctx->partials = nullptr;
         ctx->partials_bytes = 0;
         BRT_HIP(hipMalloc((void**)&ctx->partials, need), "hipMalloc(partials)");
+        BRT_HIP(hipMemset(ctx->partials, 0, need), "hipMemset(partials)");
         ctx->partials_bytes = need;
     }
+    uint32_t* done = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ctx->partials) + ctx->partials_bytes - 256);
     BRT_HIP(brt::launch_adjoint(*grid, prox->max_neighbors, *steer, dt, n,
                                 reinterpret_cast<const float4*>(sorted_pos),
                                 reinterpret_cast<const float4*>(sorted_forward),
@@ -456,9 +479,163 @@ int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid, const br
                                 reinterpret_cast<const float4*>(adj_forward_out),
                                 reinterpret_cast<const float4*>(adj_smooth_accel_out),
                                 reinterpret_cast<float4*>(adj_pos_in), reinterpret_cast<float4*>(adj_forward_in),
-                                reinterpret_cast<float4*>(adj_smooth_accel_in), ctx->partials, weight_grad,
-                                ctx->xcd_remap, (hipStream_t)stream),
+                                reinterpret_cast<float4*>(adj_smooth_accel_in), ctx->partials, done,
+                                weight_grad, ctx->xcd_remap, (hipStream_t)stream),
             "k_adjoint");
+    return BRT_OK;
+}
+
+int brt_osw_steer_forces(brt_context* ctx, const brt_prox_params* prox, const brt_steer_params* steer, float dt,
+                         uint32_t step, uint32_t n, const float* sorted_pos, const float* sorted_forward,
+                         const uint32_t* hash, const uint32_t* lists, float* wander, float* force_out,
+                         void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_prox(ctx, prox);
+    if (rc) return rc;
+    if ((rc = check_steer(ctx, steer))) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(sorted_pos && sorted_forward && hash && lists && force_out, "NULL array");
+    BRT_REQUIRE(aligned16(sorted_pos) && aligned16(sorted_forward) && aligned16(lists) && aligned16(force_out),
+                "float4 arrays must be 16-byte aligned");
+    BRT_REQUIRE(steer->wander_weight == 0.0f || wander != nullptr, "wander_weight != 0 needs the wander state");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_steer_forces(prox->max_neighbors, *steer, dt, step, n,
+                                     reinterpret_cast<const float4*>(sorted_pos),
+                                     reinterpret_cast<const float4*>(sorted_forward),
+                                     reinterpret_cast<const uint2*>(hash), lists, reinterpret_cast<float2*>(wander),
+                                     reinterpret_cast<float4*>(force_out), ctx->xcd_remap, (hipStream_t)stream),
+            "k_steer<forces>");
+    return BRT_OK;
+}
+
+int brt_osw_apply_steering_force(brt_context* ctx, const brt_grid_params* grid, const brt_steer_params* steer,
+                                 float dt, uint32_t n, const float* sorted_pos, const float* sorted_forward,
+                                 const float* sorted_smooth_accel, const uint32_t* hash, const float* force_in,
+                                 float* pos_out, float* forward_out, float* smooth_accel_out, float* force_out,
+                                 void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_grid(ctx, grid);
+    if (rc) return rc;
+    if ((rc = check_steer(ctx, steer))) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(sorted_pos && sorted_forward && sorted_smooth_accel && hash && force_in && pos_out && forward_out &&
+                    smooth_accel_out,
+                "NULL array");
+    BRT_REQUIRE(aligned16(sorted_pos) && aligned16(sorted_forward) && aligned16(sorted_smooth_accel) &&
+                    aligned16(force_in) && aligned16(pos_out) && aligned16(forward_out) &&
+                    aligned16(smooth_accel_out) && aligned16(force_out),
+                "float4 arrays must be 16-byte aligned");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_apply(*grid, *steer, dt, n, reinterpret_cast<const float4*>(sorted_pos),
+                              reinterpret_cast<const float4*>(sorted_forward),
+                              reinterpret_cast<const float4*>(sorted_smooth_accel),
+                              reinterpret_cast<const uint2*>(hash), reinterpret_cast<const float4*>(force_in),
+                              reinterpret_cast<float4*>(pos_out), reinterpret_cast<float4*>(forward_out),
+                              reinterpret_cast<float4*>(smooth_accel_out), reinterpret_cast<float4*>(force_out),
+                              (hipStream_t)stream),
+            "k_apply");
+    return BRT_OK;
+}
+
+static int check_bc(brt_context* ctx, const brt_bc_params* bc) {
+    BRT_REQUIRE(bc != nullptr, "BehaviorClustering params are NULL");
+    BRT_REQUIRE(bc->feature_count >= 1 && bc->feature_count <= BRT_MAX_FEATURES, "feature_count must be in 1..%d",
+                BRT_MAX_FEATURES);
+    return BRT_OK;
+}
+
+int brt_bc_neigh_similarities(brt_context* ctx, const brt_prox_params* prox, const brt_bc_params* bc, uint32_t n,
+                              const uint32_t* hash, const uint32_t* lists, const float* features,
+                              const float* stats_in, float* stats_out, float* similarities, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_prox(ctx, prox);
+    if (rc) return rc;
+    if ((rc = check_bc(ctx, bc))) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(hash && lists && features && similarities, "NULL array");
+    BRT_REQUIRE(!bc->adaptive || (stats_in && stats_out && aligned16(stats_in) && aligned16(stats_out)),
+                "adaptive similarities need 16-byte aligned stats_in and stats_out");
+    BRT_REQUIRE(!bc->adaptive || stats_in != stats_out, "stats_in and stats_out must differ");
+    BRT_REQUIRE(((uintptr_t)features & 3u) == 0 && ((uintptr_t)similarities & 3u) == 0, "float arrays misaligned");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_bc_similarities(prox->max_neighbors, *bc, n, reinterpret_cast<const uint2*>(hash), lists,
+                                        features, reinterpret_cast<const float4*>(stats_in),
+                                        reinterpret_cast<float4*>(stats_out), similarities, (hipStream_t)stream),
+            "k_bc_similarities");
+    return BRT_OK;
+}
+
+int brt_bc_steer_for_clustering(brt_context* ctx, const brt_prox_params* prox, const brt_steer_params* steer,
+                                const brt_bc_params* bc, uint32_t n, const float* sorted_pos,
+                                const float* sorted_forward, const uint32_t* lists, const float* similarities,
+                                const float* force_in, float* force_out, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_prox(ctx, prox);
+    if (rc) return rc;
+    if ((rc = check_steer(ctx, steer))) return rc;
+    if ((rc = check_bc(ctx, bc))) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(sorted_pos && sorted_forward && lists && similarities && force_in && force_out, "NULL array");
+    BRT_REQUIRE(aligned16(sorted_pos) && aligned16(sorted_forward) && aligned16(force_in) && aligned16(force_out),
+                "float4 arrays must be 16-byte aligned");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_bc_cluster_steer(prox->max_neighbors, *steer, *bc, n,
+                                         reinterpret_cast<const float4*>(sorted_pos),
+                                         reinterpret_cast<const float4*>(sorted_forward), lists, similarities,
+                                         reinterpret_cast<const float4*>(force_in),
+                                         reinterpret_cast<float4*>(force_out), (hipStream_t)stream),
+            "k_bc_cluster_steer");
+    return BRT_OK;
+}
+
+int brt_bc_cluster_membership(brt_context* ctx, const brt_prox_params* prox, uint32_t n, uint32_t iterations,
+                              const uint32_t* hash, const uint32_t* lists, const float* sorted_pos,
+                              const float* membership_in, float* membership_out, float* connector, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_prox(ctx, prox);
+    if (rc) return rc;
+    BRT_REQUIRE(n <= (1u << 24), "n = %u: cluster labels are sorted indices held in floats, n must be <= 2^24", n);
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(hash && lists && sorted_pos && membership_in && membership_out, "NULL array");
+    BRT_REQUIRE(aligned16(sorted_pos) && aligned16(connector), "float4 arrays must be 16-byte aligned");
+    BRT_REQUIRE(((uintptr_t)membership_in & 3u) == 0 && ((uintptr_t)membership_out & 3u) == 0,
+                "membership arrays misaligned");
+    if ((rc = use_device(ctx))) return rc;
+    float* lbl = nullptr;
+    if (iterations >= 2) {
+        if ((rc = ensure_tmp(ctx, (size_t)n * 8u))) return rc;
+        lbl = reinterpret_cast<float*>(ctx->tmp);
+    }
+    BRT_HIP(brt::launch_cluster_membership(prox->max_neighbors, n, iterations, reinterpret_cast<const uint2*>(hash),
+                                           lists, reinterpret_cast<const float4*>(sorted_pos), membership_in,
+                                           membership_out, reinterpret_cast<float4*>(connector), lbl,
+                                           lbl ? lbl + n : nullptr, (hipStream_t)stream),
+            "k_lp_iterate");
+    return BRT_OK;
+}
+
+int brt_bc_cluster_hit_rate(brt_context* ctx, const brt_prox_params* prox, float same_cluster_max_radius, uint32_t n,
+                            const uint32_t* hash, const uint32_t* lists, const float* sorted_pos,
+                            const float* membership_in, float* membership_out, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_prox(ctx, prox);
+    if (rc) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(hash && lists && sorted_pos && membership_in && membership_out, "NULL array");
+    BRT_REQUIRE(membership_in != membership_out, "membership_in and membership_out must differ");
+    BRT_REQUIRE(aligned16(sorted_pos), "sorted_pos must be 16-byte aligned");
+    BRT_REQUIRE(((uintptr_t)membership_in & 3u) == 0 && ((uintptr_t)membership_out & 3u) == 0,
+                "membership arrays misaligned");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_hit_rate(prox->max_neighbors, same_cluster_max_radius, n, reinterpret_cast<const uint2*>(hash),
+                                 lists, reinterpret_cast<const float4*>(sorted_pos), membership_in, membership_out,
+                                 (hipStream_t)stream),
+            "k_hit_rate");
     return BRT_OK;
 }
 
@@ -491,6 +668,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
     if ((rc = check_steer(ctx, steer))) return rc;
     BRT_REQUIRE(b != nullptr, "buffers are NULL");
     if ((rc = check_n(ctx, b->n))) return rc;
+    if ((rc = check_n_lists(ctx, b->n))) return rc;
     const uint32_t n = b->n;
     if (n == 0) return BRT_OK;
     BRT_REQUIRE(b->pos && b->forward && b->smooth_accel && b->sorted_pos && b->sorted_forward &&
@@ -535,7 +713,8 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
             "k_reorder_ranges");
     if (ev) BRT_HIP(hipEventRecord((*ev)[2], s), "hipEventRecord");
     const float T = brt_radius_threshold(prox->search_radius);
-    if (ctx->fused_step && !prox->use_knn) {
+    const uint32_t lds = resolve_lds_lists(ctx, prox);
+    if (resolve_fused(ctx, prox) && !prox->use_knn) {
         // phase 2: neighbour lists + steering + integration in one pass
         BRT_HIP(brt::launch_neighbors_steer(*grid, *prox, T, *steer, dt, step,
                                             reinterpret_cast<const float4*>(b->sorted_pos),
@@ -545,7 +724,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
                                             b->lists, reinterpret_cast<float2*>(b->wander),
                                             reinterpret_cast<float4*>(b->pos), reinterpret_cast<float4*>(b->forward),
                                             reinterpret_cast<float4*>(b->smooth_accel), ctx->uniform_max_cells,
-                                            ctx->xcd_remap, ctx->lds_lists, s),
+                                            ctx->xcd_remap, lds, s),
                 "k_prox<fused>");
         if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
         if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
@@ -553,7 +732,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
         // phase 2: neighbour lists
         BRT_HIP(brt::launch_neighbors(*grid, *prox, T, reinterpret_cast<const float4*>(b->sorted_pos),
                                       reinterpret_cast<const uint2*>(b->hash), b->cell_start, b->cell_end, n,
-                                      b->lists, ctx->uniform_max_cells, ctx->xcd_remap, ctx->lds_lists, s),
+                                      b->lists, ctx->uniform_max_cells, ctx->xcd_remap, lds, s),
                 "k_neighbors");
         if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
         // phase 3: fused steering + integration, scattered back to agent-ID order

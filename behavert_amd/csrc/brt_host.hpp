@@ -14,7 +14,7 @@
 struct brt_context {
     int device = 0;
     std::string err;
-    void* tmp = nullptr;        // sort ping-pong buffer (uint2[n])
+    void* tmp = nullptr;        // sort ping-pong buffer (uint2[n]); label ping-pong (float[2n])
     size_t tmp_bytes = 0;
     uint32_t* meta = nullptr;   // histograms, tile counters, look-back status
     size_t meta_bytes = 0;
@@ -25,11 +25,22 @@ struct brt_context {
     size_t events_used = 0;
     double phase_ms[BRT_NUM_PHASES] = {0, 0, 0, 0, 0};
     uint32_t phase_steps = 0;
-    int fused_step = 1;
+    int fused_step = -1;        // -1: automatic (resolve_fused)
     uint32_t uniform_max_cells = 1;
     uint32_t xcd_remap = 1;
-    uint32_t lds_lists = 1;
+    int lds_lists = -1;         // -1: automatic (resolve_lds_lists)
 };
+
+// Automatic launch shapes, from the K = 32 and K = 64 workloads measured on
+// gfx950: at K = 32 the fused neighbour + steering kernel and LDS-staged lists
+// are fastest; at K = 64 the two-pass path with register-chunk lists is (the
+// fused kernel's registers and the LDS list tile cut occupancy).
+inline bool resolve_fused(const brt_context* ctx, const brt_prox_params* p) {
+    return ctx->fused_step >= 0 ? ctx->fused_step != 0 : p->max_neighbors <= 32u;
+}
+inline uint32_t resolve_lds_lists(const brt_context* ctx, const brt_prox_params* p) {
+    return ctx->lds_lists >= 0 ? (uint32_t)(ctx->lds_lists != 0) : (p->max_neighbors <= 32u ? 1u : 0u);
+}
 
 namespace brt {
 namespace host {
@@ -39,8 +50,11 @@ int hip_err(brt_context* ctx, hipError_t e, const char* what);
 int use_device(brt_context* ctx);
 int check_grid(brt_context* ctx, const brt_grid_params* g);
 int check_n(brt_context* ctx, uint32_t n);
+// the neighbour kernels address positions with 32-bit byte offsets
+int check_n_lists(brt_context* ctx, uint32_t n);
 int check_prox(brt_context* ctx, const brt_prox_params* p);
 int check_steer(brt_context* ctx, const brt_steer_params* s);
+int ensure_tmp(brt_context* ctx, size_t bytes);
 int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan);
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }

@@ -68,13 +68,34 @@ hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_pa
                         float4* force_out, uint32_t xcd_remap, hipStream_t s);
 hipError_t launch_features(uint32_t K, float rate, uint32_t n, const uint2* hash, const uint32_t* lists,
                            const float4* fin, float4* fout, hipStream_t s);
+// the decomposed steering sequence: behaviour forces only, then the integrator
+hipError_t launch_steer_forces(uint32_t K, const brt_steer_params& sp, float dt, uint32_t step, uint32_t n,
+                               const float4* spos, const float4* sfwd, const uint2* hash, const uint32_t* lists,
+                               float2* wander, float4* force_out, uint32_t xcd_remap, hipStream_t s);
+hipError_t launch_apply(const brt_grid_params& g, const brt_steer_params& sp, float dt, uint32_t n, const float4* spos,
+                        const float4* sfwd, const float4* ssa, const uint2* hash, const float4* force_in,
+                        float4* pos_out, float4* fwd_out, float4* sa_out, float4* force_out, hipStream_t s);
+
+// BehaviorClustering similarities and cluster steering
+hipError_t launch_bc_similarities(uint32_t K, const brt_bc_params& bc, uint32_t n, const uint2* hash,
+                                  const uint32_t* lists, const float* features, const float4* stats_in,
+                                  float4* stats_out, float* sims, hipStream_t s);
+hipError_t launch_bc_cluster_steer(uint32_t K, const brt_steer_params& sp, const brt_bc_params& bc, uint32_t n,
+                                   const float4* spos, const float4* sfwd, const uint32_t* lists, const float* sims,
+                                   const float4* force_in, float4* force_out, hipStream_t s);
+hipError_t launch_cluster_membership(uint32_t K, uint32_t n, uint32_t iterations, const uint2* hash,
+                                     const uint32_t* lists, const float4* spos, const float* m_in, float* m_out,
+                                     float4* conn, float* lbl_a, float* lbl_b, hipStream_t s);
+hipError_t launch_hit_rate(uint32_t K, float radius, uint32_t n, const uint2* hash, const uint32_t* lists,
+                           const float4* spos, const float* m_in, float* m_out, hipStream_t s);
 
 // adjoint
 hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t n,
                           const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
                           const uint32_t* lists, const float4* gpos_out, const float4* gfwd_out,
                           const float4* gsa_out, float4* gpos_in, float4* gfwd_in, float4* gsa_in,
-                          float* partials, float* weight_grad, uint32_t xcd_remap, hipStream_t s);
+                          float* partials, uint32_t* done, float* weight_grad, uint32_t xcd_remap,
+                          hipStream_t s);
 
 inline uint32_t list_stride(uint32_t K) { return 4u * (K / 4u + 1u); }
 

@@ -3,7 +3,7 @@
 // The reference runs, per step, resetSteerForce + one kernel per behaviour
 // (each re-reading the neighbour list and re-gathering neighbour state and
 // read-modify-writing a double-buffered float4 force) + the integrator
-// (PlugIns/OpenSteerWrapper/include/OpenSteerWrapperPlugIn.h:287-424,
+// (PlugIns/OpenSteerWrapper/include/OpenSteerWrapperPlugIn.h:288-426,
 //  OpenSteerWrapper_kernel.cu:62-333).  Here one lane per sorted agent walks its
 // list once and keeps the three behaviour sums in registers.  The sums are still
 // formed in list order and combined into the force in the reference's order
@@ -25,7 +25,9 @@ namespace {
 
 // One lane per sorted agent: walk the list in order (8 entries per round, their
 // gathers issued together), accumulate the behaviour sums, then assemble the
-// force and integrate.
+// force and integrate (APPLY), or only store the behaviour force in sorted
+// order (the steer-force accumulator of the decomposed sequence).
+template <bool APPLY>
 __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, brt_steer_params sp, float dt,
                                                uint32_t step, uint32_t n, const float4* __restrict__ spos,
                                                const float4* __restrict__ sfwd, const float4* __restrict__ ssa,
@@ -74,7 +76,26 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
             }
         }
     }
-    finish_agent(g, sp, wts, dt, step, i, id, id, myPos, myFwd, Fw.w, xyz(SA), SA.w, acc, o);
+    if (APPLY) {
+        finish_agent(g, sp, wts, dt, step, i, id, id, myPos, myFwd, Fw.w, xyz(SA), SA.w, acc, o);
+    } else {
+        const V3 f = behaviour_force(sp, wts, dt, step, id, id, myPos, myFwd, Fw.w, acc, o.wander);
+        o.force_out[i] = make_float4(f.x, f.y, f.z, 0.0f);
+    }
+}
+
+// applySteeringForces from an accumulated force (sorted float4): the world-centre
+// seek and the integrator of the fused pass, on a force built by other passes.
+__global__ void __launch_bounds__(256) k_apply(brt_grid_params g, brt_steer_params sp, float dt, uint32_t n,
+                                               const float4* __restrict__ spos, const float4* __restrict__ sfwd,
+                                               const float4* __restrict__ ssa, const uint2* __restrict__ hash,
+                                               const float4* __restrict__ force_in, AgentOut o) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = hash[i].y;
+    if (id >= n) return;
+    const float4 Fw = sfwd[i], SA = ssa[i];
+    apply_force(g, sp, dt, i, id, xyz(force_in[i]), xyz(spos[i]), xyz(Fw), Fw.w, xyz(SA), SA.w, o);
 }
 
 __global__ void __launch_bounds__(256) k_features(uint32_t W, float rate, uint32_t n, const uint2* __restrict__ hash,
@@ -122,8 +143,27 @@ hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_pa
                         const uint32_t* lists, float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
                         float4* force_out, uint32_t xcd_remap, hipStream_t s) {
     AgentOut o{pos_out, fwd_out, sa_out, force_out, wander};
-    hipLaunchKernelGGL(k_steer, dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt, step, n, spos, sfwd,
-                       ssa, hash, lists, o, xcd_remap);
+    hipLaunchKernelGGL((k_steer<true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt, step, n,
+                       spos, sfwd, ssa, hash, lists, o, xcd_remap);
+    return hipGetLastError();
+}
+
+hipError_t launch_steer_forces(uint32_t K, const brt_steer_params& sp, float dt, uint32_t step, uint32_t n,
+                               const float4* spos, const float4* sfwd, const uint2* hash, const uint32_t* lists,
+                               float2* wander, float4* force_out, uint32_t xcd_remap, hipStream_t s) {
+    AgentOut o{nullptr, nullptr, nullptr, force_out, wander};
+    brt_grid_params g{};
+    hipLaunchKernelGGL((k_steer<false>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt, step, n,
+                       spos, sfwd, sfwd, hash, lists, o, xcd_remap);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply(const brt_grid_params& g, const brt_steer_params& sp, float dt, uint32_t n, const float4* spos,
+                        const float4* sfwd, const float4* ssa, const uint2* hash, const float4* force_in,
+                        float4* pos_out, float4* fwd_out, float4* sa_out, float4* force_out, hipStream_t s) {
+    AgentOut o{pos_out, fwd_out, sa_out, force_out, nullptr};
+    hipLaunchKernelGGL(k_apply, dim3((n + 255) / 256), dim3(256), 0, s, g, sp, dt, n, spos, sfwd, ssa, hash, force_in,
+                       o);
     return hipGetLastError();
 }
 

@@ -35,9 +35,12 @@
 #include "brt_steer_common.hpp"
 
 // Candidates tested per round of the scan loops (their loads are in flight
-// together).
+// together): per-lane (vector) rounds and wave-uniform (scalar) rounds.
 #ifndef BRT_PROX_GROUP
 #define BRT_PROX_GROUP 4
+#endif
+#ifndef BRT_PROX_GROUP_U
+#define BRT_PROX_GROUP_U 4
 #endif
 
 namespace brt {
@@ -45,6 +48,7 @@ namespace brt {
 namespace {
 
 constexpr int kGroup = BRT_PROX_GROUP;
+constexpr int kGroupU = BRT_PROX_GROUP_U;
 
 // Loads through the constant address space: with a wave-uniform index these
 // become scalar-cache loads (s_load_dwordx4).  The arrays are not written by
@@ -209,15 +213,44 @@ __device__ __forceinline__ void finish_list(Lane& L, uint32_t W) {
     row[0] = make_uint4(k, k > 0u ? L.h0 : P, k > 1u ? L.h1 : P, k > 2u ? L.h2 : P);
 }
 
-// Tests candidate c (position q) for lane L and appends it when it passes,
-// without branching.
-template <bool LDSL>
-__device__ __forceinline__ void test_candidate(const ProxArgs& a, Lane& L, uint32_t c, uint32_t lim, float4 q) {
-    const float rx = q.x - L.px, ry = q.y - L.py, rz = q.z - L.pz;
-    const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
-    const float d2 = (xx + yy) + zz;
-    const bool take = (c < lim) & (c != L.i) & !(d2 > a.T) & (L.cnt < a.K);
-    append<LDSL>(L, c, take);
+// Tests the candidates j .. j+kGroup-1 (positions q) for lane L and appends
+// those that pass, in order, without branching.  rem = lim - j (candidates
+// left in the cell), di = i - j (where the agent itself sits in the round).
+// CAP: some lane may reach K inside this round, so every append re-checks
+// the count.
+template <bool LDSL, bool CAP, int G>
+__device__ __forceinline__ void test_round(const ProxArgs& a, Lane& L, uint32_t j, uint32_t rem, uint32_t di,
+                                           const float4* q) {
+#pragma unroll
+    for (int t = 0; t < G; ++t) {
+        const float rx = q[t].x - L.px, ry = q[t].y - L.py, rz = q[t].z - L.pz;
+        const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
+        const float d2 = (xx + yy) + zz;
+        bool take = ((uint32_t)t < rem) & (di != (uint32_t)t) & !(d2 > a.T);
+        if (CAP) take = take & (L.cnt < a.K);
+        append<LDSL>(L, j + (uint32_t)t, take);
+    }
+}
+
+template <bool LDSL, int G>
+__device__ __forceinline__ void test_round_any(const ProxArgs& a, Lane& L, uint32_t j, uint32_t rem, uint32_t di,
+                                               const float4* q) {
+    if (__ballot(L.cnt + (uint32_t)G > a.K) == 0)
+        test_round<LDSL, false, G>(a, L, j, rem, di, q);
+    else
+        test_round<LDSL, true, G>(a, L, j, rem, di, q);
+}
+
+// Buffer resource over a float4 array of n elements: 32-bit offsets with an
+// immediate part, and reads past the end return zeros instead of faulting.
+// Requires n < 2^28.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t float4_rsrc(const float4* p, uint32_t n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(p), (short)0, (int)(n * 16u), 0x00020000);
+}
+
+__device__ __forceinline__ float4 bload(__amdgpu_buffer_rsrc_t r, uint32_t byte_offset) {
+    const vfloat4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_offset, 0, 0);
+    return make_float4(v.x, v.y, v.z, v.w);
 }
 
 // Uniform mode: every active lane shares cell (ux,uy,uz); control flow and
@@ -247,12 +280,11 @@ __device__ __forceinline__ void scan_uniform(const ProxArgs& a, const Seg& sg, L
                     const bool need = (L.cnt < a.K) & !(bd > a.Tskip);
                     if (__ballot(need) == 0) continue;
                 }
-                for (uint32_t j = s; j < lim; j += kGroup) {
-                    float4 q[kGroup];
+                for (uint32_t j = s; j < lim; j += kGroupU) {
+                    float4 q[kGroupU];
 #pragma unroll
-                    for (int t = 0; t < kGroup; ++t) q[t] = cload(cpos, (j + t < lim) ? j + t : lim - 1);
-#pragma unroll
-                    for (int t = 0; t < kGroup; ++t) test_candidate<LDSL>(a, L, j + t, lim, q[t]);
+                    for (int t = 0; t < kGroupU; ++t) q[t] = cload(cpos, (j + t < lim) ? j + t : lim - 1);
+                    test_round_any<LDSL, kGroupU>(a, L, j, lim - j, L.i - j, q);
                     if (__ballot(L.cnt < a.K) == 0) return;
                 }
             }
@@ -267,6 +299,7 @@ template <bool LDSL>
 __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, Lane& L, I3 gp) {
     const brt_grid_params& g = sg.g;
     const uint32_t maxBodies = g.max_bodies_per_cell;
+    const __amdgpu_buffer_rsrc_t rpos = float4_rsrc(a.spos, a.n);
 #pragma unroll
     for (int dx = -1; dx <= 1; ++dx) {
         const int cx = gp.x + dx;
@@ -291,11 +324,11 @@ __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, La
             uint32_t lim = (e - s) > maxBodies ? s + maxBodies : e;
             lim = lim < a.n ? lim : a.n;
             for (uint32_t j = s; j < lim; j += kGroup) {
+                // candidates past lim are read (zeros past n) and not taken
                 float4 q[kGroup];
 #pragma unroll
-                for (int t = 0; t < kGroup; ++t) q[t] = a.spos[(j + t < lim) ? j + t : lim - 1];
-#pragma unroll
-                for (int t = 0; t < kGroup; ++t) test_candidate<LDSL>(a, L, j + t, lim, q[t]);
+                for (int t = 0; t < kGroup; ++t) q[t] = bload(rpos, j * 16u + 16u * (uint32_t)t);
+                test_round_any<LDSL, kGroup>(a, L, j, lim - j, L.i - j, q);
                 if (L.cnt >= a.K) return;
             }
         }
@@ -378,13 +411,15 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
             acc.init();
             const Weights wts = sg.w;
             if (wts.s != 0.0f || wts.a != 0.0f || wts.c != 0.0f) {
+                const __amdgpu_buffer_rsrc_t rpos = float4_rsrc(a.spos, a.n);
+                const __amdgpu_buffer_rsrc_t rfwd = float4_rsrc(a.sfwd, a.n);
                 for (uint32_t t = 0; t < L.cnt; t += 4) {
                     float4 q[4], f[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const uint32_t j = (t + u < L.cnt) ? L.row[1 + t + u] : L.i;
-                        q[u] = a.spos[j];
-                        f[u] = a.sfwd[j];
+                        q[u] = bload(rpos, j * 16u);
+                        f[u] = bload(rfwd, j * 16u);
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {

@@ -76,15 +76,14 @@ struct AgentOut {
     float2* wander;     // by ID, in/out, optional
 };
 
-// Force assembly in the reference's order, then applySteeringForceSingle, then
-// the scatter of the new state to agent-ID order.  `id` indexes the state
-// arrays; `lid` (the ID within the agent's flock, equal to id outside batches)
-// picks the seek target and keys the wander stream.
-__device__ __forceinline__ void finish_agent(const brt_grid_params& g, const brt_steer_params& sp, Weights w,
-                                             float dt, uint32_t step, uint32_t i, uint32_t id, uint32_t lid,
-                                             V3 myPos, V3 myFwd,
-                                             float mySpeed, V3 mySa, float saW, const SteerAccum& acc,
-                                             const AgentOut& o) {
+// The behaviour forces in the reference's accumulation order (each term added
+// as term + force, the blendIntoSteeringForce order, OpenSteerWrapper_resources.cu:113-129):
+// separation, alignment, cohesion, then the build's target seek and wander.
+// `id` indexes the state arrays; `lid` (the ID within the agent's flock, equal
+// to id outside batches) picks the seek target and keys the wander stream.
+__device__ __forceinline__ V3 behaviour_force(const brt_steer_params& sp, Weights w, float dt, uint32_t step,
+                                              uint32_t id, uint32_t lid, V3 myPos, V3 myFwd, float mySpeed,
+                                              const SteerAccum& acc, float2* wander) {
     V3 force = v3(0.f, 0.f, 0.f);
     if (w.s != 0.0f) {
         V3 s = acc.nSep > 0 ? normalize(acc.sSep / (float)acc.nSep) : acc.sSep;
@@ -103,23 +102,32 @@ __device__ __forceinline__ void finish_agent(const brt_grid_params& g, const brt
         V3 s = steerForSeek(myPos, mySpeed, myFwd, v3(t[0], t[1], t[2]), sp.max_speed);
         force = s * sp.target_weight + force;
     }
-    if (sp.wander_weight != 0.0f && o.wander) {
+    if (sp.wander_weight != 0.0f && wander) {
         uint32_t c[4] = {step, 0u, 0u, 0u};
         philox4x32_10(c, sp.wander_seed, lid);
         const float u0 = u24(c[0]), u1 = u24(c[1]);
         const float walk = 12.0f * dt;
-        const float2 wsu = o.wander[id];
+        const float2 wsu = wander[id];
         float ws = wsu.x + ((u0 * 2.0f - 1.0f) * walk);
         ws = ws < -1.0f ? -1.0f : (ws > 1.0f ? 1.0f : ws);
         float wu = wsu.y + ((u1 * 2.0f - 1.0f) * walk);
         wu = wu < -1.0f ? -1.0f : (wu > 1.0f ? 1.0f : wu);
-        o.wander[id] = make_float2(ws, wu);
+        wander[id] = make_float2(ws, wu);
         const V3 cr = cross(v3(0.f, 1.f, 0.f), myFwd);
         const V3 side = dot(cr, cr) == 0.0f ? v3(1.f, 0.f, 0.f) : normalize(cr);
         const V3 up = cross(myFwd, side);
         V3 s = side * ws + up * wu;
         force = s * sp.wander_weight + force;
     }
+    return force;
+}
+
+// computeSeekingsWorldCenter (added last), then applySteeringForceSingle, then
+// the scatter of the new state to agent-ID order.  force_out (sorted, optional)
+// receives the summed force before the integrator.
+__device__ __forceinline__ void apply_force(const brt_grid_params& g, const brt_steer_params& sp, float dt,
+                                            uint32_t i, uint32_t id, V3 force, V3 myPos, V3 myFwd, float mySpeed,
+                                            V3 mySa, float saW, const AgentOut& o) {
     if (sp.seek_world_center) {
         const float cx = g.world_center[0], cy = g.world_center[1], cz = g.world_center[2];
         const float rx = g.world_radius[0], ry = g.world_radius[1], rz = g.world_radius[2];
@@ -150,6 +158,15 @@ __device__ __forceinline__ void finish_agent(const brt_grid_params& g, const brt
     o.pos_out[id] = make_float4(newPos.x, newPos.y, newPos.z, 1.0f);
     o.sa_out[id] = make_float4(mySa.x, mySa.y, mySa.z, saW);
     o.fwd_out[id] = make_float4(newFwd.x, newFwd.y, newFwd.z, mySpeed);
+}
+
+// The whole per-agent tail of the steering pass.
+__device__ __forceinline__ void finish_agent(const brt_grid_params& g, const brt_steer_params& sp, Weights w,
+                                             float dt, uint32_t step, uint32_t i, uint32_t id, uint32_t lid,
+                                             V3 myPos, V3 myFwd, float mySpeed, V3 mySa, float saW,
+                                             const SteerAccum& acc, const AgentOut& o) {
+    const V3 force = behaviour_force(sp, w, dt, step, id, lid, myPos, myFwd, mySpeed, acc, o.wander);
+    apply_force(g, sp, dt, i, id, force, myPos, myFwd, mySpeed, mySa, saW, o);
 }
 
 }  // namespace brt

@@ -108,6 +108,39 @@ class Context:
             sorted_pos.shape[0], _ptr(sorted_pos), _ptr(sorted_fwd), _ptr(sorted_sa), _ptr(hash_), _ptr(lists),
             _ptr(wander), _ptr(pos_out), _ptr(fwd_out), _ptr(sa_out), _ptr(force_out), current_stream()))
 
+    def steer_forces(self, prox, steer, dt, step, sorted_pos, sorted_fwd, hash_, lists, force_out, wander=None):
+        self.check(self._lib.brt_osw_steer_forces(
+            self.handle, C.byref(prox), C.byref(steer), float(dt), int(step), sorted_pos.shape[0], _ptr(sorted_pos),
+            _ptr(sorted_fwd), _ptr(hash_), _ptr(lists), _ptr(wander), _ptr(force_out), current_stream()))
+
+    def apply_steering_force(self, grid, steer, dt, sorted_pos, sorted_fwd, sorted_sa, hash_, force_in, pos_out,
+                             fwd_out, sa_out, force_out=None):
+        self.check(self._lib.brt_osw_apply_steering_force(
+            self.handle, C.byref(grid), C.byref(steer), float(dt), sorted_pos.shape[0], _ptr(sorted_pos),
+            _ptr(sorted_fwd), _ptr(sorted_sa), _ptr(hash_), _ptr(force_in), _ptr(pos_out), _ptr(fwd_out),
+            _ptr(sa_out), _ptr(force_out), current_stream()))
+
+    def bc_similarities(self, prox, bc, hash_, lists, features, stats_in, stats_out, sims):
+        self.check(self._lib.brt_bc_neigh_similarities(
+            self.handle, C.byref(prox), C.byref(bc), hash_.shape[0], _ptr(hash_), _ptr(lists), _ptr(features),
+            _ptr(stats_in), _ptr(stats_out), _ptr(sims), current_stream()))
+
+    def bc_steer_for_clustering(self, prox, steer, bc, sorted_pos, sorted_fwd, lists, sims, force_in, force_out):
+        self.check(self._lib.brt_bc_steer_for_clustering(
+            self.handle, C.byref(prox), C.byref(steer), C.byref(bc), sorted_pos.shape[0], _ptr(sorted_pos),
+            _ptr(sorted_fwd), _ptr(lists), _ptr(sims), _ptr(force_in), _ptr(force_out), current_stream()))
+
+    def cluster_membership(self, prox, iterations, hash_, lists, sorted_pos, membership_in, membership_out,
+                           connector=None):
+        self.check(self._lib.brt_bc_cluster_membership(
+            self.handle, C.byref(prox), hash_.shape[0], int(iterations), _ptr(hash_), _ptr(lists), _ptr(sorted_pos),
+            _ptr(membership_in), _ptr(membership_out), _ptr(connector), current_stream()))
+
+    def cluster_hit_rate(self, prox, radius, hash_, lists, sorted_pos, membership_in, membership_out):
+        self.check(self._lib.brt_bc_cluster_hit_rate(
+            self.handle, C.byref(prox), float(radius), hash_.shape[0], _ptr(hash_), _ptr(lists), _ptr(sorted_pos),
+            _ptr(membership_in), _ptr(membership_out), current_stream()))
+
     def feature_update(self, prox, rate, hash_, lists, fin, fout):
         self.check(self._lib.brt_bc_feature_update(self.handle, C.byref(prox), float(rate), hash_.shape[0],
                                                    _ptr(hash_), _ptr(lists), _ptr(fin), _ptr(fout),
@@ -116,6 +149,12 @@ class Context:
     # ------------------------------------------------------------ options
     def set_option(self, option: int, value: int):
         self.check(self._lib.brt_context_set_option(self.handle, int(option), int(value)))
+
+    def step_shape(self, prox) -> tuple[bool, bool]:
+        """(fused, lds_lists) that brt_step uses for these proximity parameters."""
+        f, l = C.c_int(), C.c_int()
+        self.check(self._lib.brt_context_step_shape(self.handle, C.byref(prox), C.byref(f), C.byref(l)))
+        return bool(f.value), bool(l.value)
 
     def get_option(self, option: int) -> int:
         v = C.c_int()

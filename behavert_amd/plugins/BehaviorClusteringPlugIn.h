@@ -1,17 +1,31 @@
-// BehaviorClusteringPlugIn.h -- per-agent feature vectors and their
-// neighbour-averaged update.
+// BehaviorClusteringPlugIn.h -- per-agent feature vectors, their
+// neighbour-averaged update and the similarity-driven cluster steering.
 //
 // The reference plug-in (PlugIns/BehaviorClustering/include/BehaviorClusteringPlugIn.h)
-// keeps a featuresCount-float vector per agent indexed by agent ID (:180-183)
-// and walks the neighbour lists with the sorted->ID map
-// (BehaviorClustering_kernel.cu:306-353).  This plug-in keeps that layout for a
-// 4-D feature (float4 by ID, double-buffered) and provides the feature update
-//   f'[id(i)] = f[id(i)] + rate * (mean_{j in list(i)} f[id(j)] - f[id(i)])
-// on the gfx950 kernel (brt_bc_feature_update).  Config section
-// [BehaviorClusteringPlugIn]: featuresCount (must be 4), featureUpdateRate.
+// keeps a featuresCount-float vector per agent indexed by agent ID (:180-183),
+// per-agent similarity statistics (min, avg, max) initialised to (1, 0.5, 0)
+// (:185-187, :594-599) and one similarity per neighbour-list slot (:223-226).
+// This plug-in keeps those layouts:
+//   features ........ float[featuresCount] by ID (single buffer)
+//   similarityStats . float4 by ID (double-buffered: read -> write, swap)
+//   neighSimilarities float[maxNeighbors + 1] per sorted agent
+// and provides
+//   computeFeatureAveraging (featuresCount 4 only):
+//     f'[id(i)] = f[id(i)] + rate * (mean_{j in list(i)} f[id(j)] - f[id(i)])
+//   steerForClustering (:740-817): naive + adaptive similarities, then cluster
+//     cohesion and alignment summed onto the steer force, between the flocking
+//     behaviours and the world-centre seek (BehaveRT_Clustering.cpp:212-235).
+// Config section [BehaviorClusteringPlugIn]: featuresCount (1..32, default 4),
+// featureUpdateRate, clusteringForceParams and alignmentClusteringForceParams
+// (".x" is the weight, :160-161, BehaviorClustering_resources.cu:339,380),
+// similarityStatsParams (three rates, BehaviorClustering_resources.cu:84-150).
+// avgSimilarityController is 1, as the reference's install() sets it (:164).
+// The reference's dataset loader, colours, cluster membership and merge/freeze
+// tools are not part of this layer; reset() draws features uniformly in [0, 1).
 #pragma once
 
 #include <random>
+#include <stdexcept>
 
 #include "OpenSteerWrapperPlugIn.h"
 
@@ -25,24 +39,32 @@ public:
 
     const std::string name() override { return "BehaviorClusteringPlugIn"; }
     const BehaveRT::DependenciesList plugInDependencies() override {
-        return BehaveRT::DependenciesList{"EnvGrid3DPlugIn", "Proximity3DPlugIn"};
+        return BehaveRT::DependenciesList{"EnvGrid3DPlugIn", "Proximity3DPlugIn", "OpenSteerWrapperPlugIn"};
     }
     void reset() override {
         Super::reset();
         std::mt19937 gen(7u);
         std::uniform_real_distribution<float> u(0.0f, 1.0f);
+        const int F = hBehaviorClusteringParams.featuresCount;
+        std::vector<float> f((size_t)F);
+        const float4 st = make_float4(1.0f, 0.5f, 0.0f, 0.0f);
         for (int i = 0; i < hBody3DParams.numBodies; ++i) {
-            float4 f = make_float4(u(gen), u(gen), u(gen), u(gen));
-            m_FeaturesVector->setHostArrayElement(i, &f);
+            for (int k = 0; k < F; ++k) f[(size_t)k] = u(gen);
+            m_FeaturesVector->setHostArrayElement(i, f.data());
+            m_SimilarityStats->setHostArrayElement(i, &st);
         }
         m_FeaturesVector->copyArrayToDevice();
         m_FeaturesVector->swapPosReadWrite();
+        m_SimilarityStats->copyArrayToDevice();
+        m_SimilarityStats->swapPosReadWrite();
     }
     void update(const float elapsedTime) override { Super::update(elapsedTime); }
 
     /// One neighbour-averaged update of every agent's feature vector.
     void computeFeatureAveraging() { computeFeatureAveraging(hBehaviorClusteringParams.featureUpdateRate); }
     void computeFeatureAveraging(float rate) {
+        if (hBehaviorClusteringParams.featuresCount != 4)
+            throw std::runtime_error("BehaviorClusteringPlugIn: the feature update supports featuresCount = 4");
         BehaveRT::DeviceInterface* di = m_CommonRes.getDeviceInterface();
         const brt_prox_params pp = this->proxParams();
         di->check(brt_bc_feature_update(di->context(), &pp, rate, hBody3DParams.numBodies,
@@ -55,7 +77,56 @@ public:
         if (di->getHostSimParams().useThreadSync) di->threadSync();
     }
 
-    BehaveRT::DeviceArrayWrapper<float4>* getFeaturesVector() { return m_FeaturesVector.get(); }
+    /// Similarity-weighted cluster cohesion and alignment, summed onto the
+    /// steer force the flocking behaviours recorded so far produce.  The
+    /// behaviours' pass uses the elapsed time of the previous
+    /// applySteeringForces (it only matters to wander); the overload takes it.
+    void steerForClustering() { steerForClustering(hOpenSteerWrapperDynParams.elapsedTime); }
+    void steerForClustering(float elapsedTime) {
+        BehaveRT::DeviceInterface* di = m_CommonRes.getDeviceInterface();
+        this->materializeSteerForce(elapsedTime);
+        const brt_prox_params pp = this->proxParams();
+        const brt_bc_params bc = bcParams();
+        const int n = hBody3DParams.numBodies;
+        di->check(brt_bc_neigh_similarities(di->context(), &pp, &bc, n,
+                                            (const uint32_t*)this->getHash()->getReadDeviceArray(),
+                                            (const uint32_t*)this->getNeighList()->getReadDeviceArray(),
+                                            m_FeaturesVector->getReadDeviceArray(),
+                                            (const float*)m_SimilarityStats->getReadDeviceArray(),
+                                            (float*)m_SimilarityStats->getWriteDeviceArray(),
+                                            m_NeighSimilarities->getReadDeviceArray(), di->stream()),
+                  "brt_bc_neigh_similarities");
+        m_SimilarityStats->swapPosReadWrite();
+        const brt_steer_params sp = this->steerParams(0u);
+        di->check(brt_bc_steer_for_clustering(di->context(), &pp, &sp, &bc, n,
+                                              (const float*)this->getPos()->getReadDeviceArray(),
+                                              (const float*)this->getForward()->getReadDeviceArray(),
+                                              (const uint32_t*)this->getNeighList()->getReadDeviceArray(),
+                                              m_NeighSimilarities->getReadDeviceArray(),
+                                              (const float*)this->getSteerForce()->getReadDeviceArray(),
+                                              (float*)this->getSteerForce()->getWriteDeviceArray(), di->stream()),
+                  "brt_bc_steer_for_clustering");
+        this->getSteerForce()->swapPosReadWrite();
+        if (di->getHostSimParams().useThreadSync) di->threadSync();
+    }
+
+    brt_bc_params bcParams() const {
+        const BehaviorClusteringParams& p = hBehaviorClusteringParams;
+        brt_bc_params bc;
+        bc.feature_count = (uint32_t)p.featuresCount;
+        bc.adaptive = 1u;
+        bc.avg_similarity_controller = p.avgSimilarityController;
+        bc.stats_params[0] = p.similarityStatsParams.x;
+        bc.stats_params[1] = p.similarityStatsParams.y;
+        bc.stats_params[2] = p.similarityStatsParams.z;
+        bc.cluster_weight = p.clusteringForceParams.x;
+        bc.alignment_weight = p.alignmentClusteringForceParams.x;
+        return bc;
+    }
+
+    BehaveRT::DeviceArrayWrapper<float>* getFeaturesVector() { return m_FeaturesVector.get(); }
+    BehaveRT::DeviceArrayWrapper<float4>* getSimilarityStats() { return m_SimilarityStats.get(); }
+    BehaveRT::DeviceArrayWrapper<float>* getNeighSimilarities() { return m_NeighSimilarities.get(); }
     int getElementsNumber() const { return hBody3DParams.numBodies; }
 
 protected:
@@ -63,19 +134,34 @@ protected:
     void install() {
         using BehaveRT::StringConverter;
         BehaveRT::UtilityConfigFile* cfg = m_CommonRes.getConfig();
+        BehaveRT::DeviceInterface* di = m_CommonRes.getDeviceInterface();
         const std::string sec = "BehaviorClusteringPlugIn";
+        BehaviorClusteringParams& p = hBehaviorClusteringParams;
         const std::string fc = cfg->getSetting("featuresCount", sec);
-        hBehaviorClusteringParams.featuresCount = fc.empty() ? 4 : StringConverter::parseInt(fc);
-        if (hBehaviorClusteringParams.featuresCount != 4)
-            throw std::runtime_error("BehaviorClusteringPlugIn: the feature update supports featuresCount = 4");
-        hBehaviorClusteringParams.featureUpdateRate =
-            StringConverter::parseFloat(cfg->getSetting("featureUpdateRate", sec));
-        m_FeaturesVector.reset(
-            new BehaveRT::DeviceArrayWrapper<float4>(m_CommonRes.getDeviceInterface(), hBody3DParams.numBodies));
+        p.featuresCount = fc.empty() ? 4 : StringConverter::parseInt(fc);
+        if (p.featuresCount < 1 || p.featuresCount > BRT_MAX_FEATURES)
+            throw std::runtime_error("BehaviorClusteringPlugIn: featuresCount must be in 1..32");
+        p.featureUpdateRate = StringConverter::parseFloat(cfg->getSetting("featureUpdateRate", sec));
+        p.clusteringForceParams = StringConverter::parseFloat3(cfg->getSetting("clusteringForceParams", sec));
+        p.alignmentClusteringForceParams =
+            StringConverter::parseFloat3(cfg->getSetting("alignmentClusteringForceParams", sec));
+        p.similarityStatsParams = StringConverter::parseFloat3(cfg->getSetting("similarityStatsParams", sec));
+        p.avgSimilarityController = 1.0f;
+        const int n = hBody3DParams.numBodies;
+        m_FeaturesVector.reset(new BehaveRT::DeviceArrayWrapper<float>(di, n, p.featuresCount));
+        m_SimilarityStats.reset(new BehaveRT::DeviceArrayWrapper<float4>(di, n));
+        m_NeighSimilarities.reset(new BehaveRT::DeviceArrayWrapper<float>(
+            di, n, (int)hProximity3DParams.maxNeighbors + 1, false));
     }
-    void uninstall() { m_FeaturesVector.reset(); }
+    void uninstall() {
+        m_FeaturesVector.reset();
+        m_SimilarityStats.reset();
+        m_NeighSimilarities.reset();
+    }
 
-    std::unique_ptr<BehaveRT::DeviceArrayWrapper<float4>> m_FeaturesVector;
+    std::unique_ptr<BehaveRT::DeviceArrayWrapper<float>> m_FeaturesVector;
+    std::unique_ptr<BehaveRT::DeviceArrayWrapper<float4>> m_SimilarityStats;
+    std::unique_ptr<BehaveRT::DeviceArrayWrapper<float>> m_NeighSimilarities;
 };
 
 }  // namespace BehaviorClustering

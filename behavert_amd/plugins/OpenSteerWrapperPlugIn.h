@@ -14,6 +14,11 @@
 // the order of BehaveRT_Flocking::update), same integrator.  A behaviour whose
 // weight is 0 is skipped, as in the reference.  The summed force is written to
 // steerForce for readers.
+// A plug-in that adds its own force between the behaviours and the world-centre
+// seek (BehaviorClustering's steerForClustering, BehaveRT_Clustering.cpp:212-235)
+// calls materializeSteerForce(): the recorded behaviours then run as their own
+// pass (brt_osw_steer_forces) into steerForce, the plug-in sums onto it, and
+// applySteeringForces integrates the result (brt_osw_apply_steering_force).
 // Extensions with no reference counterpart: setSeekTargets (multi-target seek)
 // and setWander (OpenSteer steerForWander driven by a counter-based RNG).
 // Config section [OpenSteerWrapperPlugIn]: separationParams, cohesionParams,
@@ -59,10 +64,14 @@ public:
         m_SmoothedAcceleration->swapPosReadWrite();
         m_Step = 0;
         m_Pending = 0;
+        m_ForceMaterialized = false;
     }
     void update(const float elapsedTime) override { Super::update(elapsedTime); }
 
-    void resetSteerForce() { m_Pending = 0; }
+    void resetSteerForce() {
+        m_Pending = 0;
+        m_ForceMaterialized = false;
+    }
     void computeSeparations() {
         if (hOpenSteerWrapperParams.separationParams.x != 0.0f) m_Pending |= kSep;
     }
@@ -86,61 +95,73 @@ public:
         m_WanderSeed = seed;
     }
 
+    /// Runs the behaviours recorded so far as their own pass: steerForce (read
+    /// buffer, sorted order) then holds their sum, ready for further forces.
+    void materializeSteerForce(float elapsedTime) {
+        if (m_ForceMaterialized) return;
+        BehaveRT::DeviceInterface* di = m_CommonRes.getDeviceInterface();
+        prepareWander(di);
+        const brt_steer_params sp = steerParams(m_Pending & ~kSeekCenter);
+        const brt_prox_params pp = this->proxParams();
+        di->check(brt_osw_steer_forces(di->context(), &pp, &sp, elapsedTime, m_Step, hBody3DParams.numBodies,
+                                       (const float*)this->getPos()->getReadDeviceArray(),
+                                       (const float*)m_Forward->getReadDeviceArray(),
+                                       (const uint32_t*)this->getHash()->getReadDeviceArray(),
+                                       (const uint32_t*)this->getNeighList()->getReadDeviceArray(),
+                                       m_Wander ? (float*)m_Wander->getReadDeviceArray() : nullptr,
+                                       (float*)m_SteerForce->getWriteDeviceArray(), di->stream()),
+                  "brt_osw_steer_forces");
+        m_SteerForce->swapPosReadWrite();
+        m_Pending &= kSeekCenter;
+        m_ForceMaterialized = true;
+    }
+
     /// All recorded behaviours + applySteeringForceSingle in one pass; the new
     /// position, forward and smoothed acceleration land by agent ID.
     void applySteeringForces(float elapsedTime) {
         BehaveRT::DeviceInterface* di = m_CommonRes.getDeviceInterface();
         hOpenSteerWrapperDynParams.elapsedTime = elapsedTime;
-        const OpenSteerWrapperParams& p = hOpenSteerWrapperParams;
-        brt_steer_params sp;
-        std::memset(&sp, 0, sizeof(sp));
-        const float3* b[3] = {&p.separationParams, &p.alignmentParams, &p.cohesionParams};
-        float* dst[3] = {sp.separation, sp.alignment, sp.cohesion};
-        const unsigned bits[3] = {kSep, kAli, kCoh};
-        for (int k = 0; k < 3; ++k) {
-            dst[k][0] = (m_Pending & bits[k]) ? b[k]->x : 0.0f;
-            dst[k][1] = b[k]->y;
-            dst[k][2] = b[k]->z;
-        }
-        sp.mass = p.commonMass;
-        sp.max_speed = p.commonMaxSpeed;
-        sp.max_force = p.commonMaxForce;
-        sp.sensing_min_radius = p.commonSensingMinRadius;
-        sp.seek_world_center = (m_Pending & kSeekCenter) ? 1u : 0u;
-        sp.use_2d_projection = hBody3DParams.use2DProjection ? 1u : 0u;
-        sp.num_targets = (uint32_t)m_NumTargets;
-        sp.target_weight = m_TargetWeight;
-        for (int k = 0; k < m_NumTargets; ++k) {
-            sp.targets[k][0] = m_Targets[k].x;
-            sp.targets[k][1] = m_Targets[k].y;
-            sp.targets[k][2] = m_Targets[k].z;
-        }
-        sp.wander_weight = m_WanderWeight;
-        sp.wander_seed = m_WanderSeed;
-        if (m_WanderWeight != 0.0f && !m_Wander) {
-            m_Wander.reset(new BehaveRT::DeviceArrayWrapper<float2>(di, hBody3DParams.numBodies, 1, false));
-            m_Wander->reset(0);
-        }
         const brt_grid_params& g = this->getGridParams();
-        const brt_prox_params pp = this->proxParams();
-        di->check(brt_osw_steer_integrate(
-                      di->context(), &g, &pp, &sp, elapsedTime, m_Step, hBody3DParams.numBodies,
-                      (const float*)this->getPos()->getReadDeviceArray(),
-                      (const float*)m_Forward->getReadDeviceArray(),
-                      (const float*)m_SmoothedAcceleration->getReadDeviceArray(),
-                      (const uint32_t*)this->getHash()->getReadDeviceArray(),
-                      (const uint32_t*)this->getNeighList()->getReadDeviceArray(),
-                      m_Wander ? (float*)m_Wander->getReadDeviceArray() : nullptr,
-                      (float*)this->getPos()->getWriteDeviceArray(), (float*)m_Forward->getWriteDeviceArray(),
-                      (float*)m_SmoothedAcceleration->getWriteDeviceArray(),
-                      (float*)m_SteerForce->getWriteDeviceArray(), di->stream()),
-                  "brt_osw_steer_integrate");
+        if (m_ForceMaterialized) {
+            if (m_Pending & (kSep | kAli | kCoh))
+                throw std::logic_error("OpenSteerWrapperPlugIn: behaviours recorded after the steer force was "
+                                       "materialized");
+            const brt_steer_params sp = steerParams(m_Pending);
+            di->check(brt_osw_apply_steering_force(
+                          di->context(), &g, &sp, elapsedTime, hBody3DParams.numBodies,
+                          (const float*)this->getPos()->getReadDeviceArray(),
+                          (const float*)m_Forward->getReadDeviceArray(),
+                          (const float*)m_SmoothedAcceleration->getReadDeviceArray(),
+                          (const uint32_t*)this->getHash()->getReadDeviceArray(),
+                          (const float*)m_SteerForce->getReadDeviceArray(),
+                          (float*)this->getPos()->getWriteDeviceArray(), (float*)m_Forward->getWriteDeviceArray(),
+                          (float*)m_SmoothedAcceleration->getWriteDeviceArray(),
+                          (float*)m_SteerForce->getWriteDeviceArray(), di->stream()),
+                      "brt_osw_apply_steering_force");
+        } else {
+            prepareWander(di);
+            const brt_steer_params sp = steerParams(m_Pending);
+            const brt_prox_params pp = this->proxParams();
+            di->check(brt_osw_steer_integrate(
+                          di->context(), &g, &pp, &sp, elapsedTime, m_Step, hBody3DParams.numBodies,
+                          (const float*)this->getPos()->getReadDeviceArray(),
+                          (const float*)m_Forward->getReadDeviceArray(),
+                          (const float*)m_SmoothedAcceleration->getReadDeviceArray(),
+                          (const uint32_t*)this->getHash()->getReadDeviceArray(),
+                          (const uint32_t*)this->getNeighList()->getReadDeviceArray(),
+                          m_Wander ? (float*)m_Wander->getReadDeviceArray() : nullptr,
+                          (float*)this->getPos()->getWriteDeviceArray(), (float*)m_Forward->getWriteDeviceArray(),
+                          (float*)m_SmoothedAcceleration->getWriteDeviceArray(),
+                          (float*)m_SteerForce->getWriteDeviceArray(), di->stream()),
+                      "brt_osw_steer_integrate");
+        }
         this->getPos()->swapPosReadWrite();
         m_SmoothedAcceleration->swapPosReadWrite();
         m_Forward->swapPosReadWrite();
         m_SteerForce->swapPosReadWrite();
         ++m_Step;
         m_Pending = 0;
+        m_ForceMaterialized = false;
         if (di->getHostSimParams().useThreadSync) di->threadSync();
     }
 
@@ -151,6 +172,43 @@ public:
 protected:
     friend struct BehaveRT::PlugInInstaller;
     enum : unsigned { kSep = 1, kAli = 2, kCoh = 4, kSeekCenter = 8 };
+
+    /// The C ABI's steering parameters for the behaviours in `pending`.
+    brt_steer_params steerParams(unsigned pending) const {
+        const OpenSteerWrapperParams& p = hOpenSteerWrapperParams;
+        brt_steer_params sp;
+        std::memset(&sp, 0, sizeof(sp));
+        const float3* b[3] = {&p.separationParams, &p.alignmentParams, &p.cohesionParams};
+        float* dst[3] = {sp.separation, sp.alignment, sp.cohesion};
+        const unsigned bits[3] = {kSep, kAli, kCoh};
+        for (int k = 0; k < 3; ++k) {
+            dst[k][0] = (pending & bits[k]) ? b[k]->x : 0.0f;
+            dst[k][1] = b[k]->y;
+            dst[k][2] = b[k]->z;
+        }
+        sp.mass = p.commonMass;
+        sp.max_speed = p.commonMaxSpeed;
+        sp.max_force = p.commonMaxForce;
+        sp.sensing_min_radius = p.commonSensingMinRadius;
+        sp.seek_world_center = (pending & kSeekCenter) ? 1u : 0u;
+        sp.use_2d_projection = hBody3DParams.use2DProjection ? 1u : 0u;
+        sp.num_targets = (uint32_t)m_NumTargets;
+        sp.target_weight = m_TargetWeight;
+        for (int k = 0; k < m_NumTargets; ++k) {
+            sp.targets[k][0] = m_Targets[k].x;
+            sp.targets[k][1] = m_Targets[k].y;
+            sp.targets[k][2] = m_Targets[k].z;
+        }
+        sp.wander_weight = m_WanderWeight;
+        sp.wander_seed = m_WanderSeed;
+        return sp;
+    }
+    void prepareWander(BehaveRT::DeviceInterface* di) {
+        if (m_WanderWeight != 0.0f && !m_Wander) {
+            m_Wander.reset(new BehaveRT::DeviceArrayWrapper<float2>(di, hBody3DParams.numBodies, 1, false));
+            m_Wander->reset(0);
+        }
+    }
 
     void install() {
         using BehaveRT::StringConverter;
@@ -188,6 +246,7 @@ protected:
     std::unique_ptr<BehaveRT::DeviceArrayWrapper<float4>> m_Forward, m_Up, m_SteerForce, m_SmoothedAcceleration;
     std::unique_ptr<BehaveRT::DeviceArrayWrapper<float2>> m_Wander;
     unsigned m_Pending = 0;
+    bool m_ForceMaterialized = false;
     uint32_t m_Step = 0;
     unsigned m_Seed = 0;
     int m_NumTargets = 0;

@@ -78,10 +78,15 @@ struct OpenSteerWrapperDynParams {
     float elapsedTime = 0.0f;
 };
 
-/// BehaviorClusteringParams subset used by the feature update
+/// BehaviorClusteringParams (PlugIns/BehaviorClustering/include/BehaviorClustering_kernel.cuh:40-57):
+/// the fields the feature update and the cluster steering read
 struct BehaviorClusteringParams {
     int featuresCount = 4;
     float featureUpdateRate = 0.0f;
+    float3 clusteringForceParams = make_float3(0, 0, 0);           // .x = weight
+    float3 alignmentClusteringForceParams = make_float3(0, 0, 0);  // .x = weight
+    float3 similarityStatsParams = make_float3(0, 0, 0);           // avg rate, min/max rate, memory
+    float avgSimilarityController = 1.0f;
 };
 
 inline Body3DParams hBody3DParams;

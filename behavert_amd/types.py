@@ -159,6 +159,35 @@ class StepBuffers(C.Structure):
     ]
 
 
+MAX_FEATURES = 32
+
+
+class BcParams(C.Structure):
+    """BehaviorClusteringParams subset read by the similarity and cluster-steering
+    kernels (PlugIns/BehaviorClustering/include/BehaviorClustering_kernel.cuh:40-59)."""
+    _fields_ = [
+        ("feature_count", C.c_uint32),
+        ("adaptive", C.c_uint32),
+        ("avg_similarity_controller", C.c_float),
+        ("stats_params", C.c_float * 3),
+        ("cluster_weight", C.c_float),
+        ("alignment_weight", C.c_float),
+    ]
+
+    @classmethod
+    def make(cls, feature_count, adaptive=True, stats_params=(0.1, 0.1, 0.001), cluster_weight=1.0,
+             alignment_weight=1.0, avg_similarity_controller=1.0):
+        b = cls()
+        b.feature_count = int(feature_count)
+        b.adaptive = 1 if adaptive else 0
+        b.avg_similarity_controller = float(avg_similarity_controller)
+        for k in range(3):
+            b.stats_params[k] = float(stats_params[k])
+        b.cluster_weight = float(cluster_weight)
+        b.alignment_weight = float(alignment_weight)
+        return b
+
+
 class FlockDesc(C.Structure):
     """One flock of a batch: its own grid, agent count and behaviour weights."""
     _fields_ = [("grid", GridParams), ("n", C.c_uint32), ("weights", C.c_float * 3)]

@@ -204,11 +204,85 @@ def full_stack(steps=20, warmup=3) -> dict:
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     abytes = algorithmic_bytes_per_agent(w)
-    ph = phase_table(ctx, w, abytes, fused=True)
+    fused, lds = ctx.step_shape(w.prox)
+    ph = phase_table(ctx, w, abytes, fused=fused)
     ctx.phase_timing(False)
     ctx.close()
     return {**w.describe(), "ms_per_step": round(1e3 * el / steps, 4), "agent_updates_per_s": w.n * steps / el,
-            "phases": ph, "timed_steps": steps}
+            "phases": ph, "timed_steps": steps, "step_shape": {"fused": fused, "lds_lists": lds}}
+
+
+def clustering(reps=20, features=8, lp_iterations=10) -> dict:
+    """BehaviorClustering on the 1,048,576-boid blob flock (K 32, 8 features by
+    agent ID): the passes of steerForClustering in BehaveRT_Clustering's order
+    (behaviour forces, similarities with adaptive statistics, cluster cohesion +
+    alignment, world-centre seek + integration), the local propagation
+    (10 iterations + connector) and the hit rate, each repeated `reps` times
+    between HIP events on the launch stream, on the lists and sorted state of
+    one brt_step."""
+    import numpy as np
+    import torch
+
+    from behavert_amd import workloads as W
+    from behavert_amd.engine import Context, Flock
+    from behavert_amd.types import BcParams, SteerParams, list_stride
+
+    w = W.blobs_1m()
+    n, K = w.n, w.prox.max_neighbors
+    Wd = list_stride(K)
+    ctx = Context(0)
+    fl = Flock(ctx, w.grid, w.prox, w.steer, w.state)
+    fl.step(w.dt, 0)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(0)
+    feats = torch.from_numpy(rng.standard_normal((n, features)).astype(np.float32)).cuda()
+    stats = torch.tensor([1.0, 0.5, 0.0, 0.0], device="cuda").repeat(n, 1).contiguous()
+    stats2 = torch.empty_like(stats)
+    sims = torch.empty((n, K + 1), dtype=torch.float32, device="cuda")
+    f0, f1, fo = (torch.empty((n, 4), dtype=torch.float32, device="cuda") for _ in range(3))
+    outs = [torch.empty((n, 4), dtype=torch.float32, device="cuda") for _ in range(3)]
+    memb = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    memb2 = torch.empty_like(memb)
+    conn = torch.empty((n, 2, 4), dtype=torch.float32, device="cuda")
+    sp = SteerParams.make(weights=(1.0, 0.8, 0.6), seek_world_center=True)
+    bc = BcParams.make(features, adaptive=True, cluster_weight=0.9, alignment_weight=0.7)
+    sargs = (fl.sorted_pos, fl.sorted_forward)
+    passes = {
+        "steer_forces": (lambda: ctx.steer_forces(w.prox, sp, w.dt, 0, *sargs, fl.hash, fl.lists, f0),
+                         8 + 32 + 4 * Wd + 16, "k_steer<false>"),
+        "similarities": (lambda: ctx.bc_similarities(w.prox, bc, fl.hash, fl.lists, feats, stats, stats2, sims),
+                         8 + 4 * Wd + 4 * features + 4 * (K + 1) + 32, "k_bc_similarities<true>"),
+        "cluster_steer": (lambda: ctx.bc_steer_for_clustering(w.prox, sp, bc, *sargs, fl.lists, sims, f0, f1),
+                          32 + 4 * Wd + 4 * (K + 1) + 32, "k_bc_cluster_steer"),
+        "apply": (lambda: ctx.apply_steering_force(w.grid, sp, w.dt, *sargs, fl.sorted_smooth_accel, fl.hash, f1,
+                                                   *outs, force_out=fo), 8 + 48 + 16 + 48 + 16, "k_apply"),
+        # T iterations: list + own label per iteration, then by-ID float3 in/out,
+        # own position and the connector pair
+        "membership": (lambda: ctx.cluster_membership(w.prox, lp_iterations, fl.hash, fl.lists, fl.sorted_pos, memb,
+                                                      memb2, conn),
+                       lp_iterations * (4 * Wd + 8) + 8 + 24 + 16 + 32, f"k_lp_iterate x{lp_iterations}"),
+        "hit_rate": (lambda: ctx.cluster_hit_rate(w.prox, 1.5, fl.hash, fl.lists, fl.sorted_pos, memb2, memb),
+                     8 + 4 * Wd + 16 + 12 + 12, "k_hit_rate"),
+    }
+    res = {}
+    for name, (fn, bpa, kern) in passes.items():
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = bpa * n / 1e9 / (ms / 1e3)
+        res[name] = {"ms": round(ms, 5), "bytes_per_agent": bpa, "GB/s": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": kern}
+    res["steer_for_clustering_ms"] = round(sum(res[k]["ms"] for k in ("steer_forces", "similarities",
+                                                                        "cluster_steer", "apply")), 5)
+    ctx.close()
+    return {"workload": w.name, "n": n, "K": K, "features": features, "lp_iterations": lp_iterations,
+            "reps": reps, "passes": res}
 
 
 def batched(steps=20, loop_steps=3) -> dict:
@@ -270,6 +344,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10, help="CPU baseline sample steps (0 disables)")
     ap.add_argument("--no-phases", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and batched measurements")
+    ap.add_argument("--only", choices=("adjoint", "batched", "full_stack", "clustering"),
+                    help="run one extra section alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with three 256 MiB device copies (the reference kernel for counter calibration)")
     args = ap.parse_args()
@@ -277,6 +353,12 @@ def main():
         print(json.dumps({"error": "this build runs on exactly one GPU: the step path has no multi-GPU form "
                                    "(see DESIGN.md, 'How the path would shard')", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.only:
+        print(json.dumps({args.only: {"adjoint": adjoint_ratio, "batched": batched,
+                                      "full_stack": full_stack, "clustering": clustering}[args.only]()}),
+              flush=True)
+        return
 
     import torch
 
@@ -315,6 +397,7 @@ def main():
         # the grid + neighbour phases on their own
         from behavert_amd.types import OPT_FUSED_STEP
 
+        fused_opt = ctx.get_option(OPT_FUSED_STEP)
         ctx.set_option(OPT_FUSED_STEP, 0)
         first = args.warmup + args.steps
         count = min(args.steps, 50)
@@ -323,7 +406,7 @@ def main():
         phases["unfused_step"] = phase_table(ctx, w, abytes, fused=False)
         phases["unfused_step"]["steps"] = [first, first + count]
         ctx.phase_timing(False)
-        ctx.set_option(OPT_FUSED_STEP, 1)
+        ctx.set_option(OPT_FUSED_STEP, fused_opt)
         fz = phases["fused_step"]
         dom = max((k for k in fz if isinstance(fz[k], dict) and abytes.get(k, 0) > 0), key=lambda k: fz[k]["ms"])
         roofline = {"bound": "hbm", "kernel": fz[dom]["kernel"], "phase": dom, "achieved": fz[dom]["GB/s"],
@@ -355,6 +438,7 @@ def main():
         extras["adjoint"] = adjoint_ratio()
         extras["batched"] = batched()
         extras["full_stack"] = full_stack()
+        extras["clustering"] = clustering()
     cpu = cpu_baseline(w, args.cpu_steps) if args.cpu_steps > 0 else None
     line = {
         "metric": "agent-updates/s at 1M boids (% HBM peak); fwd+adjoint vs plain step time",

@@ -115,7 +115,9 @@ const char* brt_version(void);
 
 /* Context options (launch-shape and fusion choices; none changes any result bit).
  * BRT_OPT_FUSED_STEP: brt_step builds the neighbour lists and runs steering +
- *   integration in one kernel (default 1; KNN lists always use two passes).
+ *   integration in one kernel (1) or writes the lists and steers from them (0)
+ *   (default -1 = automatic: fused for K <= 32, two passes above, where the
+ *   fused kernel's occupancy falls; KNN lists always use two passes).
  * BRT_OPT_UNIFORM_MAX_CELLS: a wave64 whose agents lie in at most this many
  *   cells scans candidates with wave-uniform (scalar-cache) loads (default 1;
  *   0 = every wave scans per lane).
@@ -124,7 +126,7 @@ const char* brt_version(void);
  * BRT_OPT_LDS_LISTS: the neighbour kernels stage each workgroup's lists in LDS
  *   and write them out with coalesced 16-byte stores (1), or gather each list's
  *   16-byte chunks in registers and store them as they fill (0), which leaves
- *   LDS free and raises occupancy.
+ *   LDS free and raises occupancy (default -1 = automatic: LDS for K <= 32).
  * No counterpart in the reference (its launch shapes are fixed per kernel:
  * Proximity3DPlugIn.h:133-134, CommonResources.cpp:78-82). */
 #define BRT_OPT_FUSED_STEP 1
@@ -133,6 +135,9 @@ const char* brt_version(void);
 #define BRT_OPT_LDS_LISTS 4
 int brt_context_set_option(brt_context* ctx, int option, int value);
 int brt_context_get_option(const brt_context* ctx, int option, int* value);
+/* The launch shape brt_step and brt_prox_neighbors use for these proximity
+ * parameters once automatic options are resolved: fused (0/1), lds_lists (0/1). */
+int brt_context_step_shape(const brt_context* ctx, const brt_prox_params* prox, int* fused, int* lds_lists);
 
 /* Host-side grid set-up: EnvGrid3DPlugIn::setupGrid
  * (PlugIns/EnvGrid3D/include/EnvGrid3DPlugIn.h:478-508).  No device work. */
@@ -186,7 +191,8 @@ int brt_envgrid_reorder_ranges(brt_context* ctx, const uint32_t* hash, uint32_t 
                                void* stream);
 /* genericCalcNeighgborhoodD (PlugIns/Proximity3D/Proximity3D_kernel.cu:89-125,
  * Proximity3D_resources.cu:59-172): capped 27-cell neighbour lists in the
- * reference's visitation order (x outermost, z innermost), W-stride layout. */
+ * reference's visitation order (x outermost, z innermost), W-stride layout.
+ * n < 2^28 (as for brt_step and batches). */
 int brt_prox_neighbors(brt_context* ctx, const brt_grid_params* grid,
                        const brt_prox_params* prox, const float* sorted_pos,
                        const uint32_t* hash, const uint32_t* cell_start,
@@ -197,7 +203,7 @@ int brt_prox_neighbors(brt_context* ctx, const brt_grid_params* grid,
 
 /* resetSteerForce + computeSeparations/Alignments/Cohesions [+ target seek,
  * wander] [+ computeSeekingsWorldCenter] + applySteeringForces fused in one
- * pass (PlugIns/OpenSteerWrapper/include/OpenSteerWrapperPlugIn.h:287-424,
+ * pass (PlugIns/OpenSteerWrapper/include/OpenSteerWrapperPlugIn.h:288-426,
  * OpenSteerWrapper_kernel.cu:62-333, OpenSteerWrapper_resources.cu:67-379).
  * Reads sorted state and lists, writes the new state by agent ID.
  * wander (float2 by ID) is read and updated in place when wander_weight != 0.
@@ -235,7 +241,103 @@ int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid,
                          float* adj_forward_in, float* adj_smooth_accel_in,
                          float* weight_grad, void* stream);
 
+/* The reference's decomposed steering sequence (OpenSteerWrapperPlugIn.h:288-426:
+ * resetSteerForce, one accumulating pass per behaviour, applySteeringForces),
+ * for callers that add forces of their own between the behaviours and the
+ * integrator (BehaviorClustering's steerForClustering does).
+ * brt_osw_steer_forces: force_out (float4 sorted) = the behaviour forces
+ * (separation, alignment, cohesion [, target seek, wander]) accumulated from 0.
+ * brt_osw_apply_steering_force: adds the world-centre seek when enabled, then
+ * integrates (applySteeringForceSingle) and writes the new state by ID;
+ * force_out (optional) receives the final force.  steer_forces followed by
+ * apply_steering_force equals brt_osw_steer_integrate bit for bit. */
+int brt_osw_steer_forces(brt_context* ctx, const brt_prox_params* prox,
+                         const brt_steer_params* steer, float dt, uint32_t step,
+                         uint32_t n, const float* sorted_pos, const float* sorted_forward,
+                         const uint32_t* hash, const uint32_t* lists, float* wander,
+                         float* force_out, void* stream);
+int brt_osw_apply_steering_force(brt_context* ctx, const brt_grid_params* grid,
+                                 const brt_steer_params* steer, float dt, uint32_t n,
+                                 const float* sorted_pos, const float* sorted_forward,
+                                 const float* sorted_smooth_accel, const uint32_t* hash,
+                                 const float* force_in, float* pos_out, float* forward_out,
+                                 float* smooth_accel_out, float* force_out, void* stream);
+
 /* ---------------------------------------------------------------- BehaviorClustering */
+
+/* BehaviorClusteringParams (PlugIns/BehaviorClustering/include/BehaviorClustering_kernel.cuh:40-59),
+ * the fields the similarity and cluster-steering kernels read. */
+#define BRT_MAX_FEATURES 32
+typedef struct brt_bc_params {
+    uint32_t feature_count;          /* featuresCount, 1..BRT_MAX_FEATURES */
+    uint32_t adaptive;               /* 1: adaptive similarities (stats update + interpolation) */
+    float avg_similarity_controller; /* avgSimilarityController (the plug-in sets 1.0) */
+    float stats_params[3];           /* similarityStatsParams: avg blend, min/max blend, memory */
+    float cluster_weight;            /* clusteringForceParams.x */
+    float alignment_weight;          /* alignmentClusteringForceParams.x */
+} brt_bc_params;
+
+/* computeNeighNaiveSimilarities, then (adaptive) computeAdaptiveSimilarities
+ * (BehaviorClustering_kernel.cu:306-353, 438-497; computeSimilarity,
+ * updateSimilarityStats, interpolateSimilarity BehaviorClustering_resources.cu:61-180),
+ * as one pass: for sorted agent i and list entry t, the cosine similarity of
+ * the two feature vectors mapped to [0, 1]; with adaptive, the agent's
+ * (min, avg, max) statistics are updated entry by entry and the similarity
+ * is interpolated through them.
+ *   features      float[n * feature_count] by agent ID
+ *   stats_in/out  float4[n] by agent ID (x min, y avg, z max, w kept); read
+ *                 and written only when adaptive (the plug-in starts them at
+ *                 (1, 0.5, 0, 0))
+ *   similarities  float[n * (K + 1)] by sorted index: entry t of agent i at
+ *                 i*(K+1) + t (the reference's neighSimilarities stride);
+ *                 entries count..K are set to 0. */
+int brt_bc_neigh_similarities(brt_context* ctx, const brt_prox_params* prox,
+                              const brt_bc_params* bc, uint32_t n, const uint32_t* hash,
+                              const uint32_t* lists, const float* features,
+                              const float* stats_in, float* stats_out,
+                              float* similarities, void* stream);
+
+/* steerForClusterCohesion then steerForClusterAlignment
+ * (BehaviorClustering_resources.cu:282-384), blended into the steer-force
+ * accumulator as the reference does: per agent
+ *   c = sum_t seek(p_t) s_t + flee(p_t) (1 - s_t);  c = normalize(c / n) * cluster_weight
+ *   a = sum_t f_t s_t;                               a = normalize(a / n - f) * alignment_weight
+ *   force_out = a + (c + force_in)     (terms are zero when the list is empty)
+ * with seek/flee = xxxsteerForSeek/Flee (OpenSteerWrapper_resources.cu:294-322).
+ * force_in/force_out float4 sorted (may alias). */
+int brt_bc_steer_for_clustering(brt_context* ctx, const brt_prox_params* prox,
+                                const brt_steer_params* steer, const brt_bc_params* bc,
+                                uint32_t n, const float* sorted_pos, const float* sorted_forward,
+                                const uint32_t* lists, const float* similarities,
+                                const float* force_in, float* force_out, void* stream);
+
+/* BehaviorClusteringPlugIn::computeClusterMembership (BehaviorClusteringPlugIn.h:823-865):
+ * local propagation of cluster labels over the neighbour lists.
+ *   reset (BehaviorClustering_kernel.cu:632-652): label of the agent at sorted
+ *     index i = (float)i;
+ *   `iterations` Jacobi steps (:583-625; the reference runs iterationLP = 10,
+ *     BehaviorClusteringPlugIn.h:683): label = min(own, listed neighbours');
+ *   connector (:657-689, optional): connector[2*id] = sorted_pos[i],
+ *     connector[2*id + 1] = sorted_pos[label].
+ *   membership_in/out  float3[n] by agent ID (x label, y hit rate, z mean
+ *                      label: y and z are carried from membership_in); may alias
+ *   connector          float4[2n] by agent ID, or NULL
+ * n <= 2^24 (labels are floats holding sorted indices exactly). */
+int brt_bc_cluster_membership(brt_context* ctx, const brt_prox_params* prox, uint32_t n,
+                              uint32_t iterations, const uint32_t* hash, const uint32_t* lists,
+                              const float* sorted_pos, const float* membership_in,
+                              float* membership_out, float* connector, void* stream);
+
+/* BehaviorClusteringPlugIn::analyzeClusterHitRate's kernel
+ * (BehaviorClustering_kernel.cu:502-578): over the listed neighbours within
+ * same_cluster_max_radius (the reference fixes 1.5, BehaviorClusteringPlugIn.h:166)
+ * of agent i, y = fraction whose label equals the agent's, z = mean of their
+ * labels (-1 when there is none); x = the agent's label.  float3 by ID;
+ * membership_in and membership_out must differ. */
+int brt_bc_cluster_hit_rate(brt_context* ctx, const brt_prox_params* prox,
+                            float same_cluster_max_radius, uint32_t n, const uint32_t* hash,
+                            const uint32_t* lists, const float* sorted_pos,
+                            const float* membership_in, float* membership_out, void* stream);
 
 /* Neighbour-averaged feature update (no GPU counterpart in the reference; the
  * list walk and the sorted->ID map follow

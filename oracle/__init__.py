@@ -14,6 +14,9 @@ restates.  Parity pins (tests/test_oracle_*.py):
   steer_f32 ........ FP64 evaluation, hand-derived 2/3-agent KATs
   philox ........... Random123 known-answer vectors
   feature_update ... closed form (uniform features are a fixed point)
+  bc_similarities .. closed forms (parallel / opposite / orthogonal features),
+                     hand-traced statistics updates, interpolation end points
+  bc_cluster_steer . two-agent KATs (similarity 1 -> seek, 0 -> flee)
 """
 from __future__ import annotations
 
@@ -23,7 +26,7 @@ import subprocess
 
 import numpy as np
 
-from behavert_amd.types import GridParams, ProxParams, SteerParams, StepBuffers, list_stride
+from behavert_amd.types import BcParams, GridParams, ProxParams, SteerParams, StepBuffers, list_stride
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
@@ -63,6 +66,16 @@ def lib():
             "orc_feature_update_f32": (None, [PP, F, U, P, P, P, P]),
             "orc_feature_update_f64": (None, [PP, F, U, P, P, P, P]),
             "orc_step_f32": (None, [GP, PP, SP, F, F, U, C.POINTER(StepBuffers), P]),
+            "orc_bc_similarities_f32": (None, [PP, C.POINTER(BcParams), U, P, P, P, P, P, P]),
+            "orc_bc_similarities_f64": (None, [PP, C.POINTER(BcParams), U, P, P, P, P, P, P]),
+            "orc_bc_cluster_steer_f32": (None, [PP, SP, C.POINTER(BcParams), U, P, P, P, P, P, P]),
+            "orc_bc_cluster_steer_f64": (None, [PP, SP, C.POINTER(BcParams), U, P, P, P, P, P, P]),
+            "orc_cluster_membership": (None, [PP, U, U, P, P, P, P, P, P]),
+            "orc_cluster_hit_rate": (None, [PP, F, U, P, P, P, P, P]),
+            "orc_steer_forces_f32": (None, [PP, SP, F, U, U, P, P, P, P, P, P]),
+            "orc_steer_forces_f64": (None, [PP, SP, F, U, U, P, P, P, P, P, P]),
+            "orc_apply_steering_force_f32": (None, [GP, SP, F, U, P, P, P, P, P, P, P, P, P]),
+            "orc_apply_steering_force_f64": (None, [GP, SP, F, U, P, P, P, P, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -197,6 +210,33 @@ def steer_f64(grid, prox, steer, dt, step, spos, sfwd, ssa, h, lists, wander=Non
                   cond=True, pred=pred)
 
 
+def steer_forces(prox, steer, dt, step, spos, sfwd, h, lists, wander=None, dtype=np.float32):
+    """The behaviours' summed steer force (sorted (n, 4)); wander (by ID) is
+    advanced in a copy, returned beside the force."""
+    n = len(spos)
+    spos = np.ascontiguousarray(spos, dtype)
+    sfwd = np.ascontiguousarray(sfwd, dtype)
+    w = None if wander is None else np.array(wander, dtype, copy=True, order="C")
+    force = np.zeros((n, 4), dtype)
+    fn = lib().orc_steer_forces_f32 if dtype == np.float32 else lib().orc_steer_forces_f64
+    fn(C.byref(prox), C.byref(steer), float(dt), int(step), n, _p(spos), _p(sfwd),
+       _p(np.ascontiguousarray(h, np.uint32)), _p(np.ascontiguousarray(lists, np.uint32)), _p(w), _p(force))
+    return force, w
+
+
+def apply_steering_force(grid, steer, dt, spos, sfwd, ssa, h, force_in, dtype=np.float32):
+    """World-centre seek + integration of a given steer force: returns
+    {pos, forward, smooth_accel (by ID), force (sorted, after the seek)}."""
+    n = len(spos)
+    arrs = [np.ascontiguousarray(a, dtype) for a in (spos, sfwd, ssa, force_in)]
+    out = {k: np.zeros((n, 4), dtype) for k in ("pos", "forward", "smooth_accel", "force")}
+    fn = lib().orc_apply_steering_force_f32 if dtype == np.float32 else lib().orc_apply_steering_force_f64
+    fn(C.byref(grid), C.byref(steer), float(dt), n, _p(arrs[0]), _p(arrs[1]), _p(arrs[2]),
+       _p(np.ascontiguousarray(h, np.uint32)), _p(arrs[3]), _p(out["pos"]), _p(out["forward"]),
+       _p(out["smooth_accel"]), _p(out["force"]))
+    return out
+
+
 def feature_update_f32(prox, rate, h, lists, fin):
     fin = np.ascontiguousarray(fin, np.float32)
     out = np.zeros_like(fin)
@@ -211,6 +251,57 @@ def feature_update_f64(prox, rate, h, lists, fin):
     lib().orc_feature_update_f64(C.byref(prox), float(rate), len(h), _p(np.ascontiguousarray(h, np.uint32)),
                                  _p(np.ascontiguousarray(lists, np.uint32)), _p(fin), _p(out))
     return out
+
+
+def bc_similarities(prox, bc, h, lists, features, stats=None, dtype=np.float32):
+    """computeNeighNaiveSimilarities (+ computeAdaptiveSimilarities when
+    bc.adaptive): returns (similarities (n, K+1) by sorted index, new stats
+    (n, 4) by ID or None)."""
+    n = len(h)
+    K = prox.max_neighbors
+    feats = np.ascontiguousarray(features, dtype)
+    st_in = np.ascontiguousarray(stats if stats is not None else np.zeros((n, 4)), dtype)
+    st_out = st_in.copy()
+    sims = np.zeros((n, K + 1), dtype)
+    fn = lib().orc_bc_similarities_f32 if dtype == np.float32 else lib().orc_bc_similarities_f64
+    fn(C.byref(prox), C.byref(bc), n, _p(np.ascontiguousarray(h, np.uint32)),
+       _p(np.ascontiguousarray(lists, np.uint32)), _p(feats), _p(st_in), _p(st_out), _p(sims))
+    return sims, (st_out if bc.adaptive else None)
+
+
+def bc_cluster_steer(prox, steer, bc, spos, sfwd, lists, sims, force_in, dtype=np.float32):
+    """steerForClusterCohesion then steerForClusterAlignment blended onto force_in
+    (sorted float4): returns the new force (n, 4)."""
+    n = len(spos)
+    arrs = [np.ascontiguousarray(a, dtype) for a in (spos, sfwd, sims, force_in)]
+    out = np.zeros((n, 4), dtype)
+    fn = lib().orc_bc_cluster_steer_f32 if dtype == np.float32 else lib().orc_bc_cluster_steer_f64
+    fn(C.byref(prox), C.byref(steer), C.byref(bc), n, _p(arrs[0]), _p(arrs[1]),
+       _p(np.ascontiguousarray(lists, np.uint32)), _p(arrs[2]), _p(arrs[3]), _p(out))
+    return out
+
+
+def cluster_membership(prox, iterations, h, lists, spos, membership=None, connector=False):
+    """computeClusterMembership: (labels float3 (n, 3) by ID, connector
+    (n, 2, 4) by ID or None); .y/.z carried from `membership` (zeros)."""
+    n = len(h)
+    m_in = np.ascontiguousarray(membership if membership is not None else np.zeros((n, 3)), np.float32)
+    m_out = np.zeros((n, 3), np.float32)
+    conn = np.zeros((n, 2, 4), np.float32) if connector else None
+    lib().orc_cluster_membership(C.byref(prox), n, int(iterations), _p(np.ascontiguousarray(h, np.uint32)),
+                                 _p(np.ascontiguousarray(lists, np.uint32)),
+                                 _p(np.ascontiguousarray(spos, np.float32)), _p(m_in), _p(m_out), _p(conn))
+    return m_out, conn
+
+
+def cluster_hit_rate(prox, radius, h, lists, spos, membership):
+    """clusterHitRate: float3 (label, hit rate, mean close label) by ID."""
+    n = len(h)
+    m_out = np.zeros((n, 3), np.float32)
+    lib().orc_cluster_hit_rate(C.byref(prox), float(radius), n, _p(np.ascontiguousarray(h, np.uint32)),
+                               _p(np.ascontiguousarray(lists, np.uint32)), _p(np.ascontiguousarray(spos, np.float32)),
+                               _p(np.ascontiguousarray(membership, np.float32)), _p(m_out))
+    return m_out
 
 
 class HostFlock:

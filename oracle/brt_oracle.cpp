@@ -293,6 +293,12 @@ struct SteerArrays {
     R* cond_out;     // sorted, optional: max over behaviours of sum|terms| / |sum|
     const float* pred_pos;  // sorted, optional: FP32 state the discrete decisions
     const float* pred_fwd;  // (radius/angle predicates, world box) are taken from
+    // decomposed steering (the reference's separate kernels, each summing onto
+    // the steer force): forces_only stops after the behaviours and writes
+    // force_out; force_in (sorted) replaces the behaviours by a given force
+    // before the world-centre seek and the integration
+    bool forces_only;
+    const R* force_in;
 };
 
 template <typename R>
@@ -315,14 +321,14 @@ void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& s
     const V3<R> myPos = ld3(A.pos, i);
     const V3<R> myFwd = ld3(A.fwd, i);
     R mySpeed = A.fwd[4 * i + 3];
-    V3<R> mySa = ld3(A.sa, i);
-    const R saW = A.sa[4 * i + 3];
+    V3<R> mySa = A.sa ? ld3(A.sa, i) : v3<R>(0, 0, 0);
+    const R saW = A.sa ? A.sa[4 * i + 3] : R(0);
     const uint32_t id = A.hash[2 * i + 1];
-    const uint32_t* L = A.lists + (size_t)i * A.W;
-    const uint32_t count = L[0];
+    const uint32_t* L = A.lists ? A.lists + (size_t)i * A.W : nullptr;
+    const uint32_t count = L ? L[0] : 0u;
     const float minR = sp.sensing_min_radius;
 
-    V3<R> force = v3<R>(0, 0, 0);
+    V3<R> force = A.force_in ? ld3(A.force_in, i) : v3<R>(0, 0, 0);
     R cond = R(0);
     auto upd_cond = [&](R mag_sum, V3<R> sum) {
         R l = length(sum);
@@ -330,94 +336,103 @@ void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& s
         if (mag_sum > R(0) && k > cond) cond = k;
     };
 
-    // genericSeparationD (OpenSteerWrapper_kernel.cu:86-105), calcSeparation (:166-211)
-    if (sp.separation[0] != 0.0f) {
-        V3<R> s = v3<R>(0, 0, 0);
-        R mag = R(0);
-        uint32_t nb = 0;
-        for (uint32_t t = 0; t < count; ++t) {
-            uint32_t j = L[1 + t];
-            V3<R> other = ld3(A.pos, j);
-            if (inBoidNeighborhood(ppos(i), ppos(j), minR, sp.separation[1], sp.separation[2], pfwd(i))) {
-                V3<R> offset = other - myPos;
-                R d2 = dot(offset, offset);
-                s = s + offset / (-d2);
-                mag = mag + R(1) / length(offset);
-                ++nb;
+    if (!A.force_in) {
+        // genericSeparationD (OpenSteerWrapper_kernel.cu:86-105), calcSeparation (:166-211)
+        if (sp.separation[0] != 0.0f) {
+            V3<R> s = v3<R>(0, 0, 0);
+            R mag = R(0);
+            uint32_t nb = 0;
+            for (uint32_t t = 0; t < count; ++t) {
+                uint32_t j = L[1 + t];
+                V3<R> other = ld3(A.pos, j);
+                if (inBoidNeighborhood(ppos(i), ppos(j), minR, sp.separation[1], sp.separation[2], pfwd(i))) {
+                    V3<R> offset = other - myPos;
+                    R d2 = dot(offset, offset);
+                    s = s + offset / (-d2);
+                    mag = mag + R(1) / length(offset);
+                    ++nb;
+                }
             }
-        }
-        if (nb > 0) {
-            if (A.cond_out) upd_cond(mag, s);
-            s = normalize(s / R(nb));
-        }
-        force = s * R(sp.separation[0]) + force;
-    }
-    // genericAlignmentD (:158-178), calcAlignment (:253-290)
-    if (sp.alignment[0] != 0.0f) {
-        V3<R> s = v3<R>(0, 0, 0);
-        R mag = R(0);
-        uint32_t nb = 0;
-        for (uint32_t t = 0; t < count; ++t) {
-            uint32_t j = L[1 + t];
-            if (inBoidNeighborhood(ppos(i), ppos(j), minR, sp.alignment[1], sp.alignment[2], pfwd(i))) {
-                s = s + ld3(A.fwd, j);
-                mag = mag + length(ld3(A.fwd, j));
-                ++nb;
+            if (nb > 0) {
+                if (A.cond_out) upd_cond(mag, s);
+                s = normalize(s / R(nb));
             }
+            force = s * R(sp.separation[0]) + force;
         }
-        if (nb > 0) {
-            if (A.cond_out) upd_cond(mag + R(nb) * length(myFwd), s - myFwd * R(nb));
-            s = normalize(s / R(nb) - myFwd);
-        }
-        force = s * R(sp.alignment[0]) + force;
-    }
-    // genericCohesionD (:62-81), calcCohesion (:216-249)
-    if (sp.cohesion[0] != 0.0f) {
-        V3<R> s = v3<R>(0, 0, 0);
-        R mag = R(0);
-        uint32_t nb = 0;
-        for (uint32_t t = 0; t < count; ++t) {
-            uint32_t j = L[1 + t];
-            V3<R> other = ld3(A.pos, j);
-            if (inBoidNeighborhood(ppos(i), ppos(j), minR, sp.cohesion[1], sp.cohesion[2], pfwd(i))) {
-                s = s + (other - myPos);
-                mag = mag + length(other - myPos);
-                ++nb;
+        // genericAlignmentD (:158-178), calcAlignment (:253-290)
+        if (sp.alignment[0] != 0.0f) {
+            V3<R> s = v3<R>(0, 0, 0);
+            R mag = R(0);
+            uint32_t nb = 0;
+            for (uint32_t t = 0; t < count; ++t) {
+                uint32_t j = L[1 + t];
+                if (inBoidNeighborhood(ppos(i), ppos(j), minR, sp.alignment[1], sp.alignment[2], pfwd(i))) {
+                    s = s + ld3(A.fwd, j);
+                    mag = mag + length(ld3(A.fwd, j));
+                    ++nb;
+                }
             }
+            if (nb > 0) {
+                if (A.cond_out) upd_cond(mag + R(nb) * length(myFwd), s - myFwd * R(nb));
+                s = normalize(s / R(nb) - myFwd);
+            }
+            force = s * R(sp.alignment[0]) + force;
         }
-        if (nb > 0) {
-            if (A.cond_out) upd_cond(mag, s);
-            s = normalize(s / R(nb));
+        // genericCohesionD (:62-81), calcCohesion (:216-249)
+        if (sp.cohesion[0] != 0.0f) {
+            V3<R> s = v3<R>(0, 0, 0);
+            R mag = R(0);
+            uint32_t nb = 0;
+            for (uint32_t t = 0; t < count; ++t) {
+                uint32_t j = L[1 + t];
+                V3<R> other = ld3(A.pos, j);
+                if (inBoidNeighborhood(ppos(i), ppos(j), minR, sp.cohesion[1], sp.cohesion[2], pfwd(i))) {
+                    s = s + (other - myPos);
+                    mag = mag + length(other - myPos);
+                    ++nb;
+                }
+            }
+            if (nb > 0) {
+                if (A.cond_out) upd_cond(mag, s);
+                s = normalize(s / R(nb));
+            }
+            force = s * R(sp.cohesion[0]) + force;
         }
-        force = s * R(sp.cohesion[0]) + force;
+        // multi-target seek (build-defined; xxxsteerForSeek toward targets[id % T])
+        if (sp.num_targets > 0 && sp.target_weight != 0.0f) {
+            const float* t = sp.targets[id % sp.num_targets];
+            V3<R> tgt = v3<R>(R(t[0]), R(t[1]), R(t[2]));
+            V3<R> s = steerForSeek(myPos, mySpeed, myFwd, tgt, R(sp.max_speed));
+            force = s * R(sp.target_weight) + force;
+        }
+        // wander (build-defined; OpenSteer steerForWander with a counter-based RNG)
+        if (sp.wander_weight != 0.0f && A.wander) {
+            uint32_t ctr[4] = {step, 0u, 0u, 0u};
+            uint32_t key[2] = {sp.wander_seed, id};
+            uint32_t r[4];
+            philox4x32_10(ctr, key, r);
+            R u0 = R(u24(r[0])), u1 = R(u24(r[1]));
+            R walk = R(12) * dt;
+            R ws = A.wander[2 * id], wu = A.wander[2 * id + 1];
+            ws = ws + ((u0 * R(2) - R(1)) * walk);
+            ws = ws < R(-1) ? R(-1) : (ws > R(1) ? R(1) : ws);
+            wu = wu + ((u1 * R(2) - R(1)) * walk);
+            wu = wu < R(-1) ? R(-1) : (wu > R(1) ? R(1) : wu);
+            A.wander[2 * id] = ws;
+            A.wander[2 * id + 1] = wu;
+            V3<R> c = cross(v3<R>(0, 1, 0), myFwd);
+            V3<R> side = dot(c, c) == R(0) ? v3<R>(1, 0, 0) : normalize(c);
+            V3<R> up = cross(myFwd, side);
+            V3<R> s = side * ws + up * wu;
+            force = s * R(sp.wander_weight) + force;
+        }
     }
-    // multi-target seek (build-defined; xxxsteerForSeek toward targets[id % T])
-    if (sp.num_targets > 0 && sp.target_weight != 0.0f) {
-        const float* t = sp.targets[id % sp.num_targets];
-        V3<R> tgt = v3<R>(R(t[0]), R(t[1]), R(t[2]));
-        V3<R> s = steerForSeek(myPos, mySpeed, myFwd, tgt, R(sp.max_speed));
-        force = s * R(sp.target_weight) + force;
-    }
-    // wander (build-defined; OpenSteer steerForWander with a counter-based RNG)
-    if (sp.wander_weight != 0.0f && A.wander) {
-        uint32_t ctr[4] = {step, 0u, 0u, 0u};
-        uint32_t key[2] = {sp.wander_seed, id};
-        uint32_t r[4];
-        philox4x32_10(ctr, key, r);
-        R u0 = R(u24(r[0])), u1 = R(u24(r[1]));
-        R walk = R(12) * dt;
-        R ws = A.wander[2 * id], wu = A.wander[2 * id + 1];
-        ws = ws + ((u0 * R(2) - R(1)) * walk);
-        ws = ws < R(-1) ? R(-1) : (ws > R(1) ? R(1) : ws);
-        wu = wu + ((u1 * R(2) - R(1)) * walk);
-        wu = wu < R(-1) ? R(-1) : (wu > R(1) ? R(1) : wu);
-        A.wander[2 * id] = ws;
-        A.wander[2 * id + 1] = wu;
-        V3<R> c = cross(v3<R>(0, 1, 0), myFwd);
-        V3<R> side = dot(c, c) == R(0) ? v3<R>(1, 0, 0) : normalize(c);
-        V3<R> up = cross(myFwd, side);
-        V3<R> s = side * ws + up * wu;
-        force = s * R(sp.wander_weight) + force;
+    if (A.forces_only) {
+        A.force_out[4 * i] = force.x;
+        A.force_out[4 * i + 1] = force.y;
+        A.force_out[4 * i + 2] = force.z;
+        A.force_out[4 * i + 3] = R(0);
+        return;
     }
     // genericSeekingWorldCenterD<false> (OpenSteerWrapper_kernel.cu:219-259)
     if (sp.seek_world_center) {
@@ -494,6 +509,142 @@ void feature_agent(uint32_t i, R rate, const uint32_t* hash, const uint32_t* lis
 
 // ------------------------------------------------------------------ sort
 
+// ------------------------------------------------------------------ BehaviorClustering
+
+// cutil_math clamp(f, a, b) = fmaxf(a, fminf(f, b)) (IEEE minNum/maxNum: a NaN
+// operand yields the other one)
+template <typename R>
+inline R cutil_clamp(R f, R a, R b) { return std::fmax(a, std::fmin(f, b)); }
+
+// BehaviorClustering_resources.cu:61-80
+template <typename R>
+inline R linear_interp(R x, R xa, R ya, R xb, R yb) {
+    return ya * ((x - xb) / (xa - xb)) + yb * ((x - xa) / (xb - xa));
+}
+template <typename R>
+inline R interpolate_similarity(R v, R mn, R avg, R mx) {
+    if (v > avg) return cutil_clamp(linear_interp(v, avg, R(0.5f), mx, R(1)), R(0), R(1));
+    return cutil_clamp(linear_interp(v, mn, R(0), avg, R(0.5f)), R(0), R(1));
+}
+
+// BehaviorClustering_resources.cu:84-150; st = (min, avg, max)
+template <typename R>
+inline void update_similarity_stats(R sim, R st[3], const brt_bc_params& bc) {
+    R mn = st[0], avg = st[1], mx = st[2];
+    const R ctrl = (R)bc.avg_similarity_controller;
+    const R blend = sim * ctrl + ((mx + mn) / R(2)) * (R(1) - ctrl);
+    avg = avg + (blend - st[1]) * (R)bc.stats_params[0];
+    avg = cutil_clamp(avg, R(0), R(1));
+    if (sim < mn) {
+        mn = mn + (sim - mn) * (R)bc.stats_params[1];
+        if (mn > avg) mn = cutil_clamp(mn, R(0), sim);
+    } else if (mn < avg) {
+        mn = mn + (R)bc.stats_params[2];
+    }
+    if (sim > mx) {
+        mx = mx + (sim - mx) * (R)bc.stats_params[1];
+        if (mx < avg) mx = cutil_clamp(mx, sim, R(1));
+    } else if (mx > avg) {
+        mx = mx - (R)bc.stats_params[2];
+    }
+    st[0] = mn;
+    st[1] = avg;
+    st[2] = mx;
+}
+
+// computeNeighNaiveSimilarities + computeAdaptiveSimilarities for sorted agent i
+// (BehaviorClustering_kernel.cu:306-353, 438-497; computeSimilarityFromTex
+// BehaviorClustering_resources.cu:180-198)
+template <typename R>
+void bc_similarity_agent(uint32_t i, uint32_t n, uint32_t K, uint32_t W, const brt_bc_params& bc,
+                         const uint32_t* hash, const uint32_t* lists, const R* features, const R* stats_in,
+                         R* stats_out, R* sims) {
+    const uint32_t F = bc.feature_count;
+    const uint32_t id = hash[2 * i + 1];
+    const uint32_t* row = lists + (size_t)i * W;
+    const uint32_t count = std::min(row[0], K);
+    R* out = sims + (size_t)i * (K + 1);
+    for (uint32_t t = 0; t <= K; ++t) out[t] = R(0);
+    if (id >= n) return;
+    const R* fi = features + (size_t)id * F;
+    R st[3] = {R(0), R(0), R(0)};
+    if (bc.adaptive)
+        for (int k = 0; k < 3; ++k) st[k] = stats_in[4 * (size_t)id + k];
+    for (uint32_t t = 0; t < count; ++t) {
+        const uint32_t j = row[1 + t];
+        const uint32_t jid = j < n ? hash[2 * j + 1] : n;
+        R sim = R(0);
+        if (jid < n) {
+            const R* fj = features + (size_t)jid * F;
+            R n1 = R(0), n2 = R(0), d = R(0);
+            for (uint32_t k = 0; k < F; ++k) {
+                d = d + fi[k] * fj[k];
+                n1 = n1 + fi[k] * fi[k];
+                n2 = n2 + fj[k] * fj[k];
+            }
+            n1 = rsqrt_sqrt(n1);
+            n2 = rsqrt_sqrt(n2);
+            sim = (d / (n1 * n2) + R(1)) / R(2);
+            if (bc.adaptive) {
+                update_similarity_stats(sim, st, bc);
+                sim = interpolate_similarity(sim, st[0], st[1], st[2]);
+            }
+        }
+        out[t] = sim;
+    }
+    if (bc.adaptive) {
+        for (int k = 0; k < 3; ++k) stats_out[4 * (size_t)id + k] = st[k];
+        stats_out[4 * (size_t)id + 3] = stats_in[4 * (size_t)id + 3];
+    }
+}
+
+// OpenSteerWrapper_resources.cu:311-322
+template <typename R>
+inline V3<R> steerForFlee(V3<R> myPos, R mySpeed, V3<R> myForward, V3<R> target, R maxSpeed) {
+    V3<R> offset = myPos - target;
+    V3<R> desired = truncateLength(offset, maxSpeed);
+    V3<R> myVel = myForward * mySpeed;
+    return desired - myVel;
+}
+
+// steerForClusterCohesion then steerForClusterAlignment for sorted agent i
+// (BehaviorClustering_resources.cu:282-384), blended as term + force.
+template <typename R>
+void bc_cluster_agent(uint32_t i, uint32_t n, uint32_t K, uint32_t W, const brt_steer_params& sp,
+                      const brt_bc_params& bc, const R* spos, const R* sfwd, const uint32_t* lists, const R* sims,
+                      const R* force_in, R* force_out) {
+    const V3<R> myPos = v3<R>(spos[4 * i], spos[4 * i + 1], spos[4 * i + 2]);
+    const V3<R> myFwd = v3<R>(sfwd[4 * i], sfwd[4 * i + 1], sfwd[4 * i + 2]);
+    const R mySpeed = sfwd[4 * i + 3];
+    const R maxS = (R)sp.max_speed;
+    const uint32_t* row = lists + (size_t)i * W;
+    const uint32_t count = std::min(row[0], K);
+    const R* s = sims + (size_t)i * (K + 1);
+    V3<R> coh = v3<R>(0, 0, 0), ali = coh;
+    for (uint32_t t = 0; t < count; ++t) {
+        const uint32_t j = row[1 + t];
+        if (j >= n) continue;
+        const R sim = s[t];
+        const V3<R> op = v3<R>(spos[4 * j], spos[4 * j + 1], spos[4 * j + 2]);
+        const V3<R> of = v3<R>(sfwd[4 * j], sfwd[4 * j + 1], sfwd[4 * j + 2]);
+        const V3<R> seek = steerForSeek(myPos, mySpeed, myFwd, op, maxS);
+        const V3<R> flee = steerForFlee(myPos, mySpeed, myFwd, op, maxS);
+        coh = coh + (seek * sim + flee * (R(1) - sim));
+        ali = ali + of * sim;
+    }
+    if (count > 0) {
+        coh = normalize(coh / (R)count) * (R)bc.cluster_weight;
+        ali = normalize(ali / (R)count - myFwd) * (R)bc.alignment_weight;
+    }
+    const V3<R> f0 = v3<R>(force_in[4 * i], force_in[4 * i + 1], force_in[4 * i + 2]);
+    const V3<R> f1 = coh + f0;
+    const V3<R> f2 = ali + f1;
+    force_out[4 * i] = f2.x;
+    force_out[4 * i + 1] = f2.y;
+    force_out[4 * i + 2] = f2.z;
+    force_out[4 * i + 3] = R(0);
+}
+
 void radix_sort_pairs(uint32_t* pairs, uint32_t n, uint32_t num_cells, std::vector<uint32_t>& tmp) {
     uint32_t bits = 0;
     while (bits < 32 && (num_cells - 1u) >> bits) ++bits;
@@ -546,6 +697,34 @@ void radix_sort_pairs(uint32_t* pairs, uint32_t n, uint32_t num_cells, std::vect
 }  // namespace
 
 // ====================================================================== C ABI
+
+// The behaviours alone (separation, alignment, cohesion, targets, wander):
+// force_out (sorted) holds their sum, as the reference's steer force holds it
+// after computeCohesions (OpenSteerWrapperPlugIn.h:288-375).
+template <typename R>
+void steer_forces(const brt_prox_params* p, const brt_steer_params* s, float dt, uint32_t step, uint32_t n,
+                  const R* spos, const R* sfwd, const uint32_t* hash, const uint32_t* lists, R* wander,
+                  R* force_out) {
+    set_threads();
+    brt_grid_params g{};
+    SteerArrays<R> A{spos, sfwd, nullptr, hash, lists, listStride(p->max_neighbors), wander, nullptr, nullptr,
+                     nullptr, force_out, nullptr, nullptr, nullptr, true, nullptr};
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) steer_agent<R>((uint32_t)i, g, *s, R(dt), step, A);
+}
+
+// computeSeekingsWorldCenter + applySteeringForces on a given steer force
+// (OpenSteerWrapperPlugIn.h:380-419).
+template <typename R>
+void apply_steering_force(const brt_grid_params* g, const brt_steer_params* s, float dt, uint32_t n,
+                          const R* spos, const R* sfwd, const R* ssa, const uint32_t* hash, const R* force_in,
+                          R* pos_out, R* fwd_out, R* sa_out, R* force_out) {
+    set_threads();
+    SteerArrays<R> A{spos, sfwd, ssa, hash, nullptr, 0, nullptr, pos_out, fwd_out, sa_out, force_out, nullptr,
+                     nullptr, nullptr, false, force_in};
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) steer_agent<R>((uint32_t)i, *g, *s, R(dt), 0, A);
+}
 
 extern "C" {
 
@@ -756,7 +935,7 @@ void orc_steer_f32(const brt_grid_params* g, const brt_prox_params* p, const brt
                    float* pos_out, float* fwd_out, float* sa_out, float* force_out) {
     set_threads();
     SteerArrays<float> A{spos, sfwd, ssa, hash, lists, listStride(p->max_neighbors),
-                         wander, pos_out, fwd_out, sa_out, force_out, nullptr, nullptr, nullptr};
+                         wander, pos_out, fwd_out, sa_out, force_out, nullptr, nullptr, nullptr, false, nullptr};
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; ++i) steer_agent<float>((uint32_t)i, *g, *s, dt, step, A);
 }
@@ -768,9 +947,33 @@ void orc_steer_f64(const brt_grid_params* g, const brt_prox_params* p, const brt
                    const float* pred_pos, const float* pred_fwd) {
     set_threads();
     SteerArrays<double> A{spos, sfwd, ssa, hash, lists, listStride(p->max_neighbors),
-                          wander, pos_out, fwd_out, sa_out, force_out, cond_out, pred_pos, pred_fwd};
+                          wander, pos_out, fwd_out, sa_out, force_out, cond_out, pred_pos, pred_fwd, false,
+                          nullptr};
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; ++i) steer_agent<double>((uint32_t)i, *g, *s, (double)dt, step, A);
+}
+
+void orc_steer_forces_f32(const brt_prox_params* p, const brt_steer_params* s, float dt, uint32_t step, uint32_t n,
+                          const float* spos, const float* sfwd, const uint32_t* hash, const uint32_t* lists,
+                          float* wander, float* force_out) {
+    steer_forces<float>(p, s, dt, step, n, spos, sfwd, hash, lists, wander, force_out);
+}
+void orc_steer_forces_f64(const brt_prox_params* p, const brt_steer_params* s, float dt, uint32_t step, uint32_t n,
+                          const double* spos, const double* sfwd, const uint32_t* hash, const uint32_t* lists,
+                          double* wander, double* force_out) {
+    steer_forces<double>(p, s, dt, step, n, spos, sfwd, hash, lists, wander, force_out);
+}
+void orc_apply_steering_force_f32(const brt_grid_params* g, const brt_steer_params* s, float dt, uint32_t n,
+                                  const float* spos, const float* sfwd, const float* ssa, const uint32_t* hash,
+                                  const float* force_in, float* pos_out, float* fwd_out, float* sa_out,
+                                  float* force_out) {
+    apply_steering_force<float>(g, s, dt, n, spos, sfwd, ssa, hash, force_in, pos_out, fwd_out, sa_out, force_out);
+}
+void orc_apply_steering_force_f64(const brt_grid_params* g, const brt_steer_params* s, float dt, uint32_t n,
+                                  const double* spos, const double* sfwd, const double* ssa, const uint32_t* hash,
+                                  const double* force_in, double* pos_out, double* fwd_out, double* sa_out,
+                                  double* force_out) {
+    apply_steering_force<double>(g, s, dt, n, spos, sfwd, ssa, hash, force_in, pos_out, fwd_out, sa_out, force_out);
 }
 
 void orc_feature_update_f32(const brt_prox_params* p, float rate, uint32_t n, const uint32_t* hash,
@@ -788,6 +991,129 @@ void orc_feature_update_f64(const brt_prox_params* p, float rate, uint32_t n, co
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; ++i)
         feature_agent<double>((uint32_t)i, (double)rate, hash, lists, W, fin, fout);
+}
+
+void orc_bc_similarities_f32(const brt_prox_params* p, const brt_bc_params* bc, uint32_t n, const uint32_t* hash,
+                             const uint32_t* lists, const float* features, const float* stats_in,
+                             float* stats_out, float* sims) {
+    set_threads();
+    const uint32_t K = p->max_neighbors, W = listStride(K);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+        bc_similarity_agent<float>((uint32_t)i, n, K, W, *bc, hash, lists, features, stats_in, stats_out, sims);
+}
+
+void orc_bc_similarities_f64(const brt_prox_params* p, const brt_bc_params* bc, uint32_t n, const uint32_t* hash,
+                             const uint32_t* lists, const double* features, const double* stats_in,
+                             double* stats_out, double* sims) {
+    set_threads();
+    const uint32_t K = p->max_neighbors, W = listStride(K);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+        bc_similarity_agent<double>((uint32_t)i, n, K, W, *bc, hash, lists, features, stats_in, stats_out, sims);
+}
+
+void orc_bc_cluster_steer_f32(const brt_prox_params* p, const brt_steer_params* sp, const brt_bc_params* bc,
+                              uint32_t n, const float* spos, const float* sfwd, const uint32_t* lists,
+                              const float* sims, const float* force_in, float* force_out) {
+    set_threads();
+    const uint32_t K = p->max_neighbors, W = listStride(K);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+        bc_cluster_agent<float>((uint32_t)i, n, K, W, *sp, *bc, spos, sfwd, lists, sims, force_in, force_out);
+}
+
+void orc_bc_cluster_steer_f64(const brt_prox_params* p, const brt_steer_params* sp, const brt_bc_params* bc,
+                              uint32_t n, const double* spos, const double* sfwd, const uint32_t* lists,
+                              const double* sims, const double* force_in, double* force_out) {
+    set_threads();
+    const uint32_t K = p->max_neighbors, W = listStride(K);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i)
+        bc_cluster_agent<double>((uint32_t)i, n, K, W, *sp, *bc, spos, sfwd, lists, sims, force_in, force_out);
+}
+
+// BehaviorClusteringPlugIn::computeClusterMembership (BehaviorClusteringPlugIn.h:823-865)
+// as the reference runs it: float3 labels by agent ID in a double buffer;
+// resetClusterMembership (BehaviorClustering_kernel.cu:632-652) writes the
+// sorted index, each computeClusterMembership_kernel pass (:583-625) the least
+// label of the agent and its listed neighbours (gathered through the
+// sorted->ID map), CMConnection_kernel (:657-689) the position pairs.
+void orc_cluster_membership(const brt_prox_params* p, uint32_t n, uint32_t iterations, const uint32_t* hash,
+                            const uint32_t* lists, const float* spos, const float* m_in, float* m_out,
+                            float* connector) {
+    set_threads();
+    const uint32_t W = listStride(p->max_neighbors);
+    std::vector<float> rd(m_in, m_in + 3 * (size_t)n), wr(3 * (size_t)n);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const uint32_t id = hash[2 * i + 1];
+        wr[3 * id] = (float)(uint32_t)i;
+        wr[3 * id + 1] = rd[3 * id + 1];
+        wr[3 * id + 2] = rd[3 * id + 2];
+    }
+    std::swap(rd, wr);
+    for (uint32_t t = 0; t < iterations; ++t) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            const uint32_t id = hash[2 * i + 1];
+            const uint32_t* L = lists + (size_t)i * W;
+            float mn = rd[3 * id];
+            for (uint32_t q = 0; q < L[0]; ++q) {
+                const float other = rd[3 * (size_t)hash[2 * (size_t)L[1 + q] + 1]];
+                if (mn > other) mn = other;
+            }
+            wr[3 * id] = mn;
+            wr[3 * id + 1] = rd[3 * id + 1];
+            wr[3 * id + 2] = rd[3 * id + 2];
+        }
+        std::swap(rd, wr);
+    }
+    std::copy(rd.begin(), rd.end(), m_out);
+    if (!connector) return;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const uint32_t id = hash[2 * i + 1];
+        const uint32_t leader = (uint32_t)rd[3 * id];
+        for (int c = 0; c < 4; ++c) {
+            connector[8 * (size_t)id + c] = spos[4 * i + c];
+            connector[8 * (size_t)id + 4 + c] = spos[4 * (size_t)leader + c];
+        }
+    }
+}
+
+// BehaviorClustering_clusterHitRate (BehaviorClustering_kernel.cu:502-578).
+void orc_cluster_hit_rate(const brt_prox_params* p, float radius, uint32_t n, const uint32_t* hash,
+                          const uint32_t* lists, const float* spos, const float* m_in, float* m_out) {
+    set_threads();
+    const uint32_t W = listStride(p->max_neighbors);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const uint32_t id = hash[2 * i + 1];
+        const uint32_t* L = lists + (size_t)i * W;
+        const float my = m_in[3 * (size_t)id];
+        const V3<float> myPos = ld3(spos, (uint32_t)i);
+        int close = 0;
+        float real = 0.0f, hit = 0.0f;
+        for (uint32_t q = 0; q < L[0]; ++q) {
+            const uint32_t j = L[1 + q];
+            const float other = m_in[3 * (size_t)hash[2 * (size_t)j + 1]];
+            const V3<float> offset = ld3(spos, j) - myPos;
+            if (length(offset) > radius) continue;
+            ++close;
+            real += other;
+            if (my == other) hit += 1.0f;
+        }
+        if (close > 0) {
+            hit /= (float)close;
+            real /= close;
+        } else {
+            real = -1.0f;
+        }
+        m_out[3 * (size_t)id] = my;
+        m_out[3 * (size_t)id + 1] = hit;
+        m_out[3 * (size_t)id + 2] = real;
+    }
 }
 
 // Whole FP32 step on host buffers laid out as brt_step_buffers (host pointers).

@@ -158,20 +158,21 @@ def test_step_fused_equals_decomposed_and_oracle_over_10_steps(ctx, orc):
 @pytest.mark.parametrize("make", [W.plain_4096, W.blobs_1m])
 def test_fused_step_equals_unfused_step(ctx, make):
     """brt_step's fused neighbour+steering kernel and the two-kernel path, each
-    with LDS-staged and register-chunk lists, give the same bits (lists, hash,
-    state) over 3 steps."""
+    with LDS-staged and register-chunk lists, and the automatic choice give the
+    same bits (lists, hash, state) over 3 steps."""
     from behavert_amd.types import OPT_FUSED_STEP, OPT_LDS_LISTS
     w = make()
     res = []
-    for fused, lds in ((1, 1), (0, 1), (1, 0), (0, 0)):
+    old = ctx.get_option(OPT_FUSED_STEP), ctx.get_option(OPT_LDS_LISTS)
+    for fused, lds in ((1, 1), (0, 1), (1, 0), (0, 0), (-1, -1)):
         ctx.set_option(OPT_FUSED_STEP, fused)
         ctx.set_option(OPT_LDS_LISTS, lds)
         try:
             fl = run_gpu_steps(ctx, w, 3)
             res.append((fl.state(), fl.grid_outputs()))
         finally:
-            ctx.set_option(OPT_FUSED_STEP, 1)
-            ctx.set_option(OPT_LDS_LISTS, 1)
+            ctx.set_option(OPT_FUSED_STEP, old[0])
+            ctx.set_option(OPT_LDS_LISTS, old[1])
     sa, ga = res[0]
     for sb, gb in res[1:]:
         for k in ("pos", "forward", "smooth_accel"):
@@ -245,3 +246,23 @@ def test_error_paths_return_codes(ctx):
     fl = Flock(ctx, w.grid, w.prox, sp, w.state)
     with pytest.raises(BrtError):
         fl.step(w.dt, 0)
+
+
+def test_automatic_step_shape(ctx):
+    """Automatic launch shape: fused + LDS lists up to K = 32, two passes with
+    register-chunk lists above; explicit options override; KNN never fuses."""
+    from behavert_amd.types import OPT_FUSED_STEP, OPT_LDS_LISTS
+    old = ctx.get_option(OPT_FUSED_STEP), ctx.get_option(OPT_LDS_LISTS)
+    try:
+        ctx.set_option(OPT_FUSED_STEP, -1)
+        ctx.set_option(OPT_LDS_LISTS, -1)
+        assert ctx.step_shape(ProxParams.make(16, 4.0)) == (True, True)
+        assert ctx.step_shape(ProxParams.make(32, 4.0)) == (True, True)
+        assert ctx.step_shape(ProxParams.make(64, 4.0)) == (False, False)
+        assert ctx.step_shape(ProxParams.make(16, 4.0, use_knn=True)) == (False, True)
+        ctx.set_option(OPT_FUSED_STEP, 1)
+        ctx.set_option(OPT_LDS_LISTS, 1)
+        assert ctx.step_shape(ProxParams.make(64, 4.0)) == (True, True)
+    finally:
+        ctx.set_option(OPT_FUSED_STEP, old[0])
+        ctx.set_option(OPT_LDS_LISTS, old[1])

@@ -72,3 +72,48 @@ def test_plugin_chain_matches_oracle(tmp_path, orc):
         hf.step(w.steer, w.dt, i)
     for a, k in zip(got, ("pos", "forward", "smooth_accel")):
         assert np.array_equal(a.view(np.uint32), getattr(hf, k).view(np.uint32)), k
+
+
+@pytest.mark.gpu
+def test_clustering_plugin_chain_matches_oracle(tmp_path, orc):
+    """BehaveRT_Clustering's update sequence through the plug-in API
+    (BehaviorClusteringPlugIn::steerForClustering between the flocking
+    behaviours and the world-centre seek) against the oracle's passes run in
+    the same order, bit for bit after 5 steps."""
+    from behavert_amd.types import BcParams, SteerParams
+
+    from ._parity import oracle_grid_outputs
+
+    if not os.path.exists(B.CLUSTERING_BIN):
+        pytest.fail("clustering example not built (run __graft_entry__.build())")
+    cfg = os.path.join(os.path.dirname(B.CLUSTERING_SRC), "BehaveRT_Clustering.cfg")
+    w = W.plain_4096()
+    F = 6
+    rng = np.random.default_rng(5)
+    feats = rng.uniform(0, 1, (w.n, F)).astype(np.float32)
+    stats = np.tile(np.array([1.0, 0.5, 0.0, 0.0], np.float32), (w.n, 1))
+    init = tmp_path / "init.bin"
+    out = tmp_path / "out.bin"
+    with open(init, "wb") as f:
+        for a in (w.state["pos"], w.state["forward"], w.state["smooth_accel"], feats, stats):
+            f.write(np.ascontiguousarray(a, np.float32).tobytes())
+    steps = 5
+    r = subprocess.run([B.CLUSTERING_BIN, cfg, str(steps), str(init), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, np.float32).reshape(5, w.n, 4)
+
+    sp = SteerParams.make(weights=(1.0, 0.8, 0.6), seek_world_center=True)
+    bc = BcParams.make(F, adaptive=True, stats_params=(0.1, 0.1, 0.001), cluster_weight=0.9, alignment_weight=0.7)
+    st = {k: np.array(w.state[k], np.float32) for k in ("pos", "forward", "smooth_accel")}
+    for _ in range(steps):
+        o = oracle_grid_outputs(orc, w.grid, w.prox, st)
+        f0, _ = orc.steer_forces(w.prox, sp, w.dt, 0, o["sorted_pos"], o["sorted_forward"], o["hash"], o["lists"])
+        sims, stats = orc.bc_similarities(w.prox, bc, o["hash"], o["lists"], feats, stats)
+        f1 = orc.bc_cluster_steer(w.prox, sp, bc, o["sorted_pos"], o["sorted_forward"], o["lists"], sims, f0)
+        res = orc.apply_steering_force(w.grid, sp, w.dt, o["sorted_pos"], o["sorted_forward"],
+                                       o["sorted_smooth_accel"], o["hash"], f1)
+        st = {k: res[k] for k in ("pos", "forward", "smooth_accel")}
+    ref = (st["pos"], st["forward"], st["smooth_accel"], stats, res["force"])
+    for a, b, k in zip(got, ref, ("pos", "forward", "smooth_accel", "stats", "force")):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k
