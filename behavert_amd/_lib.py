@@ -49,7 +49,12 @@ SIGNATURES = {
     "brt_osw_apply_steering_force": (_I, [_P, _GP, _SP, _F, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "brt_bc_neigh_similarities": (_I, [_P, _PP, C.POINTER(BcParams), _U32, _P, _P, _P, _P, _P, _P, _P]),
     "brt_bc_steer_for_clustering": (_I, [_P, _PP, _SP, C.POINTER(BcParams), _U32, _P, _P, _P, _P, _P, _P, _P]),
+    "brt_bc_similarities_steer": (_I, [_P, _PP, _SP, C.POINTER(BcParams), _U32, _P, _P, _P, _P, _P, _P, _P, _P,
+                                       _P, _P, _P]),
     "brt_bc_cluster_membership": (_I, [_P, _PP, _U32, _U32, _P, _P, _P, _P, _P, _P, _P]),
+    "brt_osw_steer_and_slow_to_avoid_neighbors": (_I, [_P, _PP, _SP, _U32, _P, _P, _P, _P, _P, _P, _P]),
+    "brt_draw_neighborhood_color": (_I, [_P, _PP, C.POINTER(_F), _U32, _U32, _P, _P, _P, _P]),
+    "brt_shapes_move_towards_target": (_I, [_P, _I, C.POINTER(_F), _U32, _P, _P, _P, _P, _P, _P]),
     "brt_bc_cluster_hit_rate": (_I, [_P, _PP, _F, _U32, _P, _P, _P, _P, _P, _P]),
     "brt_osw_adjoint_step": (_I, [_P, _GP, _PP, _SP, _F, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "brt_step": (_I, [_P, _GP, _PP, _SP, _F, _F, _U32, C.POINTER(StepBuffers), _P]),

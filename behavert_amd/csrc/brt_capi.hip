@@ -561,10 +561,45 @@ int brt_bc_neigh_similarities(brt_context* ctx, const brt_prox_params* prox, con
     BRT_REQUIRE(!bc->adaptive || stats_in != stats_out, "stats_in and stats_out must differ");
     BRT_REQUIRE(((uintptr_t)features & 3u) == 0 && ((uintptr_t)similarities & 3u) == 0, "float arrays misaligned");
     if ((rc = use_device(ctx))) return rc;
+    if ((rc = ensure_tmp(ctx, brt::bc_scratch_bytes(bc->feature_count, n)))) return rc;
     BRT_HIP(brt::launch_bc_similarities(prox->max_neighbors, *bc, n, reinterpret_cast<const uint2*>(hash), lists,
                                         features, reinterpret_cast<const float4*>(stats_in),
-                                        reinterpret_cast<float4*>(stats_out), similarities, (hipStream_t)stream),
+                                        reinterpret_cast<float4*>(stats_out), similarities, ctx->tmp, nullptr,
+                                        nullptr, nullptr, nullptr, nullptr, (hipStream_t)stream),
             "k_bc_similarities");
+    return BRT_OK;
+}
+
+int brt_bc_similarities_steer(brt_context* ctx, const brt_prox_params* prox, const brt_steer_params* steer,
+                              const brt_bc_params* bc, uint32_t n, const uint32_t* hash, const uint32_t* lists,
+                              const float* features, const float* stats_in, float* stats_out, float* similarities,
+                              const float* sorted_pos, const float* sorted_forward, const float* force_in,
+                              float* force_out, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_prox(ctx, prox);
+    if (rc) return rc;
+    if ((rc = check_steer(ctx, steer))) return rc;
+    if ((rc = check_bc(ctx, bc))) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(hash && lists && features && similarities && sorted_pos && sorted_forward && force_in && force_out,
+                "NULL array");
+    BRT_REQUIRE(!bc->adaptive || (stats_in && stats_out && aligned16(stats_in) && aligned16(stats_out)),
+                "adaptive similarities need 16-byte aligned stats_in and stats_out");
+    BRT_REQUIRE(!bc->adaptive || stats_in != stats_out, "stats_in and stats_out must differ");
+    BRT_REQUIRE(((uintptr_t)features & 3u) == 0 && ((uintptr_t)similarities & 3u) == 0, "float arrays misaligned");
+    BRT_REQUIRE(aligned16(sorted_pos) && aligned16(sorted_forward) && aligned16(force_in) && aligned16(force_out),
+                "float4 arrays must be 16-byte aligned");
+    if ((rc = use_device(ctx))) return rc;
+    if ((rc = ensure_tmp(ctx, brt::bc_scratch_bytes(bc->feature_count, n)))) return rc;
+    BRT_HIP(brt::launch_bc_similarities(prox->max_neighbors, *bc, n, reinterpret_cast<const uint2*>(hash), lists,
+                                        features, reinterpret_cast<const float4*>(stats_in),
+                                        reinterpret_cast<float4*>(stats_out), similarities, ctx->tmp, steer,
+                                        reinterpret_cast<const float4*>(sorted_pos),
+                                        reinterpret_cast<const float4*>(sorted_forward),
+                                        reinterpret_cast<const float4*>(force_in),
+                                        reinterpret_cast<float4*>(force_out), (hipStream_t)stream),
+            "k_bc_similarities<steer>");
     return BRT_OK;
 }
 
@@ -632,10 +667,77 @@ int brt_bc_cluster_hit_rate(brt_context* ctx, const brt_prox_params* prox, float
     BRT_REQUIRE(((uintptr_t)membership_in & 3u) == 0 && ((uintptr_t)membership_out & 3u) == 0,
                 "membership arrays misaligned");
     if ((rc = use_device(ctx))) return rc;
+    if ((rc = ensure_tmp(ctx, (size_t)n * 4u))) return rc;
     BRT_HIP(brt::launch_hit_rate(prox->max_neighbors, same_cluster_max_radius, n, reinterpret_cast<const uint2*>(hash),
                                  lists, reinterpret_cast<const float4*>(sorted_pos), membership_in, membership_out,
-                                 (hipStream_t)stream),
+                                 reinterpret_cast<float*>(ctx->tmp), (hipStream_t)stream),
             "k_hit_rate");
+    return BRT_OK;
+}
+
+int brt_osw_steer_and_slow_to_avoid_neighbors(brt_context* ctx, const brt_prox_params* prox,
+                                              const brt_steer_params* steer, uint32_t n, const float* sorted_pos,
+                                              const float* sorted_forward, const uint32_t* lists,
+                                              float* forward_out, const float* force_in, float* force_out,
+                                              void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_prox(ctx, prox);
+    if (rc) return rc;
+    if ((rc = check_steer(ctx, steer))) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(sorted_pos && sorted_forward && lists && forward_out && force_in && force_out, "NULL array");
+    BRT_REQUIRE(aligned16(sorted_pos) && aligned16(sorted_forward) && aligned16(lists) && aligned16(forward_out) &&
+                    aligned16(force_in) && aligned16(force_out),
+                "float4 arrays and lists must be 16-byte aligned");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_slow_avoid(prox->max_neighbors, *steer, n, reinterpret_cast<const float4*>(sorted_pos),
+                                   reinterpret_cast<const float4*>(sorted_forward), lists,
+                                   reinterpret_cast<float4*>(forward_out), reinterpret_cast<const float4*>(force_in),
+                                   reinterpret_cast<float4*>(force_out), (hipStream_t)stream),
+            "k_slow_avoid");
+    return BRT_OK;
+}
+
+int brt_draw_neighborhood_color(brt_context* ctx, const brt_prox_params* prox, const float color_base[3],
+                                uint32_t vertexes_per_agent, uint32_t n, const uint32_t* hash, const uint32_t* lists,
+                                float* color, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_prox(ctx, prox);
+    if (rc) return rc;
+    if ((rc = check_n(ctx, n))) return rc;
+    BRT_REQUIRE(color_base != nullptr, "color_base is NULL");
+    BRT_REQUIRE(vertexes_per_agent == 1u || vertexes_per_agent == 3u, "vertexes_per_agent must be 1 or 3");
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(hash && lists && color, "NULL array");
+    BRT_REQUIRE(aligned16(color), "color must be 16-byte aligned");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_neighborhood_color(prox->max_neighbors, color_base, vertexes_per_agent, n,
+                                           reinterpret_cast<const uint2*>(hash), lists,
+                                           reinterpret_cast<float4*>(color), (hipStream_t)stream),
+            "k_neighborhood_color");
+    return BRT_OK;
+}
+
+int brt_shapes_move_towards_target(brt_context* ctx, int index_shift, const float target_base[3], uint32_t n,
+                                   const uint32_t* hash, const float* sorted_pos, const float* targets,
+                                   const float* force_in, float* force_out, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = check_n(ctx, n);
+    if (rc) return rc;
+    BRT_REQUIRE(target_base != nullptr, "target_base is NULL");
+    BRT_REQUIRE(index_shift >= 0 && (n == 0 || (uint32_t)index_shift < n), "index_shift must be in [0, n)");
+    if (n == 0) return BRT_OK;
+    BRT_REQUIRE(hash && sorted_pos && targets && force_in && force_out, "NULL array");
+    BRT_REQUIRE(aligned16(sorted_pos) && aligned16(targets) && aligned16(force_in) && aligned16(force_out),
+                "float4 arrays must be 16-byte aligned");
+    if ((rc = use_device(ctx))) return rc;
+    BRT_HIP(brt::launch_move_towards_target(index_shift, target_base, n, reinterpret_cast<const uint2*>(hash),
+                                            reinterpret_cast<const float4*>(sorted_pos),
+                                            reinterpret_cast<const float4*>(targets),
+                                            reinterpret_cast<const float4*>(force_in),
+                                            reinterpret_cast<float4*>(force_out), (hipStream_t)stream),
+            "k_move_towards_target");
     return BRT_OK;
 }
 

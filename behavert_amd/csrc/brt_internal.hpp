@@ -77,17 +77,33 @@ hipError_t launch_apply(const brt_grid_params& g, const brt_steer_params& sp, fl
                         float4* pos_out, float4* fwd_out, float4* sa_out, float4* force_out, hipStream_t s);
 
 // BehaviorClustering similarities and cluster steering
+// scratch: bc_scratch_bytes(F, n), 16-byte aligned (features in sorted order + norms);
+// sp != nullptr also runs the cluster steering on the fly (force_in -> force_out)
+size_t bc_scratch_bytes(uint32_t feature_count, uint32_t n);
 hipError_t launch_bc_similarities(uint32_t K, const brt_bc_params& bc, uint32_t n, const uint2* hash,
                                   const uint32_t* lists, const float* features, const float4* stats_in,
-                                  float4* stats_out, float* sims, hipStream_t s);
+                                  float4* stats_out, float* sims, void* scratch, const brt_steer_params* sp,
+                                  const float4* spos, const float4* sfwd, const float4* force_in, float4* force_out,
+                                  hipStream_t s);
 hipError_t launch_bc_cluster_steer(uint32_t K, const brt_steer_params& sp, const brt_bc_params& bc, uint32_t n,
                                    const float4* spos, const float4* sfwd, const uint32_t* lists, const float* sims,
                                    const float4* force_in, float4* force_out, hipStream_t s);
 hipError_t launch_cluster_membership(uint32_t K, uint32_t n, uint32_t iterations, const uint2* hash,
                                      const uint32_t* lists, const float4* spos, const float* m_in, float* m_out,
                                      float4* conn, float* lbl_a, float* lbl_b, hipStream_t s);
+// lbl: float[n] scratch (the labels in sorted order)
 hipError_t launch_hit_rate(uint32_t K, float radius, uint32_t n, const uint2* hash, const uint32_t* lists,
-                           const float4* spos, const float* m_in, float* m_out, hipStream_t s);
+                           const float4* spos, const float* m_in, float* m_out, float* lbl, hipStream_t s);
+
+// list consumers outside the step (brt_consumers.hip)
+hipError_t launch_slow_avoid(uint32_t K, const brt_steer_params& sp, uint32_t n, const float4* spos,
+                             const float4* sfwd, const uint32_t* lists, float4* fwd_out, const float4* force_in,
+                             float4* force_out, hipStream_t s);
+hipError_t launch_neighborhood_color(uint32_t K, const float base[3], uint32_t vertexes, uint32_t n,
+                                     const uint2* hash, const uint32_t* lists, float4* color, hipStream_t s);
+hipError_t launch_move_towards_target(int shift, const float base[3], uint32_t n, const uint2* hash,
+                                      const float4* spos, const float4* targets, const float4* force_in,
+                                      float4* force_out, hipStream_t s);
 
 // adjoint
 hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t n,

@@ -130,6 +130,13 @@ class Context:
             self.handle, C.byref(prox), C.byref(steer), C.byref(bc), sorted_pos.shape[0], _ptr(sorted_pos),
             _ptr(sorted_fwd), _ptr(lists), _ptr(sims), _ptr(force_in), _ptr(force_out), current_stream()))
 
+    def bc_similarities_steer(self, prox, steer, bc, hash_, lists, features, stats_in, stats_out, sims,
+                              sorted_pos, sorted_fwd, force_in, force_out):
+        self.check(self._lib.brt_bc_similarities_steer(
+            self.handle, C.byref(prox), C.byref(steer), C.byref(bc), hash_.shape[0], _ptr(hash_), _ptr(lists),
+            _ptr(features), _ptr(stats_in), _ptr(stats_out), _ptr(sims), _ptr(sorted_pos), _ptr(sorted_fwd),
+            _ptr(force_in), _ptr(force_out), current_stream()))
+
     def cluster_membership(self, prox, iterations, hash_, lists, sorted_pos, membership_in, membership_out,
                            connector=None):
         self.check(self._lib.brt_bc_cluster_membership(
@@ -140,6 +147,24 @@ class Context:
         self.check(self._lib.brt_bc_cluster_hit_rate(
             self.handle, C.byref(prox), float(radius), hash_.shape[0], _ptr(hash_), _ptr(lists), _ptr(sorted_pos),
             _ptr(membership_in), _ptr(membership_out), current_stream()))
+
+    def steer_and_slow_to_avoid_neighbors(self, prox, steer, sorted_pos, sorted_fwd, lists, forward_out, force_in,
+                                          force_out):
+        self.check(self._lib.brt_osw_steer_and_slow_to_avoid_neighbors(
+            self.handle, C.byref(prox), C.byref(steer), sorted_pos.shape[0], _ptr(sorted_pos), _ptr(sorted_fwd),
+            _ptr(lists), _ptr(forward_out), _ptr(force_in), _ptr(force_out), current_stream()))
+
+    def neighborhood_color(self, prox, color_base, vertexes, hash_, lists, color):
+        base = (C.c_float * 3)(*[float(x) for x in color_base])
+        self.check(self._lib.brt_draw_neighborhood_color(self.handle, C.byref(prox), base, int(vertexes),
+                                                         hash_.shape[0], _ptr(hash_), _ptr(lists), _ptr(color),
+                                                         current_stream()))
+
+    def move_towards_target(self, index_shift, target_base, hash_, sorted_pos, targets, force_in, force_out):
+        base = (C.c_float * 3)(*[float(x) for x in target_base])
+        self.check(self._lib.brt_shapes_move_towards_target(self.handle, int(index_shift), base, hash_.shape[0],
+                                                            _ptr(hash_), _ptr(sorted_pos), _ptr(targets),
+                                                            _ptr(force_in), _ptr(force_out), current_stream()))
 
     def feature_update(self, prox, rate, hash_, lists, fin, fout):
         self.check(self._lib.brt_bc_feature_update(self.handle, C.byref(prox), float(rate), hash_.shape[0],

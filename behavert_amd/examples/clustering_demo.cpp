@@ -1,17 +1,19 @@
 // clustering_demo.cpp -- a BehaveRT_Clustering-style driver on the plug-in layer.
 //
 // The per-step call sequence is that of BehaveRT_Clustering's update
-// (Samples/OpensteerDemo/plugins/BehaveRT_Clustering.cpp:209-235), with GL,
-// cluster-membership propagation and merge/freeze calls removed:
+// (Samples/OpensteerDemo/plugins/BehaveRT_Clustering.cpp:209-235), with the GL
+// and merge/freeze calls removed:
 //   reorderSimData, computeNeighborhoods, resetSteerForce, computeSeparations,
 //   computeAlignments, computeCohesions, steerForClustering,
-//   [computeSeekingsWorldCenter], applySteeringForces(dt)
+//   [computeClusterMembership], [computeSeekingsWorldCenter], applySteeringForces(dt)
+// ([Demo] computeLocalPropagation and seekWorldCenter switch the bracketed calls.)
 // usage: clustering_demo <config.cfg> <steps> [init.bin] [out.bin]
 //   init.bin: pos, forward, smooth_accel as float32 [N][4] each, features as
 //             float32 [N][featuresCount], similarity stats as float32 [N][4],
 //             all in agent-ID order (without it the plug-ins' reset() state is used)
 //   out.bin:  pos, forward, smooth_accel, similarity stats (ID order), then the
-//             last step's steer force (sorted order), float32 [N][4] each.
+//             last step's steer force (sorted order), float32 [N][4] each, then
+//             the cluster membership float32 [N][3] (ID order).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -48,6 +50,9 @@ static bool write_state(const char* path, ClusteringEngine& e) {
         a->copyArrayFromDevice();
         out.write(reinterpret_cast<const char*>(a->getHostArray()), (std::streamsize)a->getBytesCount());
     }
+    BehaveRT::DeviceArrayWrapper<float>* m = e.getClusterMembership();
+    m->copyArrayFromDevice();
+    out.write(reinterpret_cast<const char*>(m->getHostArray()), (std::streamsize)m->getBytesCount());
     return (bool)out;
 }
 
@@ -63,6 +68,8 @@ int main(int argc, char** argv) {
             m_CommonRes.getConfig()->getSetting("updateElapsedTime", "Benchmarking"));
         const bool seekCenter = BehaveRT::StringConverter::parseBool(
             m_CommonRes.getConfig()->getSetting("seekWorldCenter", "Demo"));
+        const bool localPropagation = BehaveRT::StringConverter::parseBool(
+            m_CommonRes.getConfig()->getSetting("computeLocalPropagation", "Demo"));
         ClusteringEngine engine;
         engine.reset();
         if (argc > 3 && !read_state(argv[3], engine)) {
@@ -77,6 +84,7 @@ int main(int argc, char** argv) {
             engine.computeAlignments();
             engine.computeCohesions();
             engine.steerForClustering(dt);
+            if (localPropagation) engine.computeClusterMembership();
             if (seekCenter) engine.computeSeekingsWorldCenter();
             engine.applySteeringForces(dt);
         }

@@ -20,8 +20,11 @@
 // (".x" is the weight, :160-161, BehaviorClustering_resources.cu:339,380),
 // similarityStatsParams (three rates, BehaviorClustering_resources.cu:84-150).
 // avgSimilarityController is 1, as the reference's install() sets it (:164).
-// The reference's dataset loader, colours, cluster membership and merge/freeze
-// tools are not part of this layer; reset() draws features uniformly in [0, 1).
+//   computeClusterMembership / analyzeClusterHitRate (:823-896): local label
+//     propagation (iterationLP = 10) with its connector, and the hit rate
+//     (sameClusterMaxRadius = 1.5), clusterMembership float3 by ID.
+// The reference's dataset loader, colours and merge/freeze tools are not part
+// of this layer; reset() draws features uniformly in [0, 1).
 #pragma once
 
 #include <random>
@@ -87,26 +90,56 @@ public:
         this->materializeSteerForce(elapsedTime);
         const brt_prox_params pp = this->proxParams();
         const brt_bc_params bc = bcParams();
-        const int n = hBody3DParams.numBodies;
-        di->check(brt_bc_neigh_similarities(di->context(), &pp, &bc, n,
+        const brt_steer_params sp = this->steerParams(0u);
+        // similarities, adaptive statistics and cluster cohesion + alignment in one pass
+        di->check(brt_bc_similarities_steer(di->context(), &pp, &sp, &bc, hBody3DParams.numBodies,
                                             (const uint32_t*)this->getHash()->getReadDeviceArray(),
                                             (const uint32_t*)this->getNeighList()->getReadDeviceArray(),
                                             m_FeaturesVector->getReadDeviceArray(),
                                             (const float*)m_SimilarityStats->getReadDeviceArray(),
                                             (float*)m_SimilarityStats->getWriteDeviceArray(),
-                                            m_NeighSimilarities->getReadDeviceArray(), di->stream()),
-                  "brt_bc_neigh_similarities");
+                                            m_NeighSimilarities->getReadDeviceArray(),
+                                            (const float*)this->getPos()->getReadDeviceArray(),
+                                            (const float*)this->getForward()->getReadDeviceArray(),
+                                            (const float*)this->getSteerForce()->getReadDeviceArray(),
+                                            (float*)this->getSteerForce()->getWriteDeviceArray(), di->stream()),
+                  "brt_bc_similarities_steer");
         m_SimilarityStats->swapPosReadWrite();
-        const brt_steer_params sp = this->steerParams(0u);
-        di->check(brt_bc_steer_for_clustering(di->context(), &pp, &sp, &bc, n,
-                                              (const float*)this->getPos()->getReadDeviceArray(),
-                                              (const float*)this->getForward()->getReadDeviceArray(),
-                                              (const uint32_t*)this->getNeighList()->getReadDeviceArray(),
-                                              m_NeighSimilarities->getReadDeviceArray(),
-                                              (const float*)this->getSteerForce()->getReadDeviceArray(),
-                                              (float*)this->getSteerForce()->getWriteDeviceArray(), di->stream()),
-                  "brt_bc_steer_for_clustering");
         this->getSteerForce()->swapPosReadWrite();
+        if (di->getHostSimParams().useThreadSync) di->threadSync();
+    }
+
+    /// Local propagation of cluster labels (BehaviorClusteringPlugIn.h:823-865):
+    /// reset to sorted indices, iterationLP min-label iterations, connector.
+    void computeClusterMembership() {
+        BehaveRT::DeviceInterface* di = m_CommonRes.getDeviceInterface();
+        const brt_prox_params pp = this->proxParams();
+        di->check(brt_bc_cluster_membership(di->context(), &pp, hBody3DParams.numBodies,
+                                            (uint32_t)hBehaviorClusteringParams.iterationLP,
+                                            (const uint32_t*)this->getHash()->getReadDeviceArray(),
+                                            (const uint32_t*)this->getNeighList()->getReadDeviceArray(),
+                                            (const float*)this->getPos()->getReadDeviceArray(),
+                                            m_ClusterMembership->getReadDeviceArray(),
+                                            m_ClusterMembership->getWriteDeviceArray(),
+                                            (float*)m_CMConnector->getReadDeviceArray(), di->stream()),
+                  "brt_bc_cluster_membership");
+        m_ClusterMembership->swapPosReadWrite();
+        if (di->getHostSimParams().useThreadSync) di->threadSync();
+    }
+    /// Per-agent hit rate and mean label of the close neighbours
+    /// (analyzeClusterHitRate, BehaviorClusteringPlugIn.h:884-896).
+    void analyzeClusterHitRate() {
+        BehaveRT::DeviceInterface* di = m_CommonRes.getDeviceInterface();
+        const brt_prox_params pp = this->proxParams();
+        di->check(brt_bc_cluster_hit_rate(di->context(), &pp, hBehaviorClusteringParams.sameClusterMaxRadius,
+                                          hBody3DParams.numBodies,
+                                          (const uint32_t*)this->getHash()->getReadDeviceArray(),
+                                          (const uint32_t*)this->getNeighList()->getReadDeviceArray(),
+                                          (const float*)this->getPos()->getReadDeviceArray(),
+                                          m_ClusterMembership->getReadDeviceArray(),
+                                          m_ClusterMembership->getWriteDeviceArray(), di->stream()),
+                  "brt_bc_cluster_hit_rate");
+        m_ClusterMembership->swapPosReadWrite();
         if (di->getHostSimParams().useThreadSync) di->threadSync();
     }
 
@@ -127,6 +160,9 @@ public:
     BehaveRT::DeviceArrayWrapper<float>* getFeaturesVector() { return m_FeaturesVector.get(); }
     BehaveRT::DeviceArrayWrapper<float4>* getSimilarityStats() { return m_SimilarityStats.get(); }
     BehaveRT::DeviceArrayWrapper<float>* getNeighSimilarities() { return m_NeighSimilarities.get(); }
+    /// float3 (label, hit rate, mean close label) by ID
+    BehaveRT::DeviceArrayWrapper<float>* getClusterMembership() { return m_ClusterMembership.get(); }
+    BehaveRT::DeviceArrayWrapper<float4>* getCMConnector() { return m_CMConnector.get(); }
     int getElementsNumber() const { return hBody3DParams.numBodies; }
 
 protected:
@@ -152,16 +188,26 @@ protected:
         m_SimilarityStats.reset(new BehaveRT::DeviceArrayWrapper<float4>(di, n));
         m_NeighSimilarities.reset(new BehaveRT::DeviceArrayWrapper<float>(
             di, n, (int)hProximity3DParams.maxNeighbors + 1, false));
+        // iterationLP and sameClusterMaxRadius are fixed by the reference's install (:166, :683)
+        p.iterationLP = 10;
+        p.sameClusterMaxRadius = 1.5f;
+        m_ClusterMembership.reset(new BehaveRT::DeviceArrayWrapper<float>(di, n, 3));
+        m_ClusterMembership->reset(0);
+        m_CMConnector.reset(new BehaveRT::DeviceArrayWrapper<float4>(di, 2 * n, 1, false));
     }
     void uninstall() {
         m_FeaturesVector.reset();
         m_SimilarityStats.reset();
         m_NeighSimilarities.reset();
+        m_ClusterMembership.reset();
+        m_CMConnector.reset();
     }
 
     std::unique_ptr<BehaveRT::DeviceArrayWrapper<float>> m_FeaturesVector;
     std::unique_ptr<BehaveRT::DeviceArrayWrapper<float4>> m_SimilarityStats;
     std::unique_ptr<BehaveRT::DeviceArrayWrapper<float>> m_NeighSimilarities;
+    std::unique_ptr<BehaveRT::DeviceArrayWrapper<float>> m_ClusterMembership;
+    std::unique_ptr<BehaveRT::DeviceArrayWrapper<float4>> m_CMConnector;
 };
 
 }  // namespace BehaviorClustering

@@ -87,6 +87,8 @@ struct BehaviorClusteringParams {
     float3 alignmentClusteringForceParams = make_float3(0, 0, 0);  // .x = weight
     float3 similarityStatsParams = make_float3(0, 0, 0);           // avg rate, min/max rate, memory
     float avgSimilarityController = 1.0f;
+    float sameClusterMaxRadius = 1.5f;
+    int iterationLP = 10;  // local propagation's iterations
 };
 
 inline Body3DParams hBody3DParams;

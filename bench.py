@@ -251,9 +251,13 @@ def clustering(reps=20, features=8, lp_iterations=10) -> dict:
         "steer_forces": (lambda: ctx.steer_forces(w.prox, sp, w.dt, 0, *sargs, fl.hash, fl.lists, f0),
                          8 + 32 + 4 * Wd + 16, "k_steer<false>"),
         "similarities": (lambda: ctx.bc_similarities(w.prox, bc, fl.hash, fl.lists, feats, stats, stats2, sims),
-                         8 + 4 * Wd + 4 * features + 4 * (K + 1) + 32, "k_bc_similarities<true>"),
+                         8 + 4 * Wd + 4 * features + 4 * (K + 1) + 32, "k_bc_sort_features+k_bc_similarities<true>"),
         "cluster_steer": (lambda: ctx.bc_steer_for_clustering(w.prox, sp, bc, *sargs, fl.lists, sims, f0, f1),
                           32 + 4 * Wd + 4 * (K + 1) + 32, "k_bc_cluster_steer"),
+        "similarities_steer": (lambda: ctx.bc_similarities_steer(w.prox, sp, bc, fl.hash, fl.lists, feats, stats,
+                                                                 stats2, sims, *sargs, f0, f1),
+                               8 + 4 * Wd + 4 * features + 4 * (K + 1) + 32 + 32 + 32,
+                               "k_bc_sort_features+k_bc_similarities<true,true>"),
         "apply": (lambda: ctx.apply_steering_force(w.grid, sp, w.dt, *sargs, fl.sorted_smooth_accel, fl.hash, f1,
                                                    *outs, force_out=fo), 8 + 48 + 16 + 48 + 16, "k_apply"),
         # T iterations: list + own label per iteration, then by-ID float3 in/out,
@@ -278,11 +282,64 @@ def clustering(reps=20, features=8, lp_iterations=10) -> dict:
         gbs = bpa * n / 1e9 / (ms / 1e3)
         res[name] = {"ms": round(ms, 5), "bytes_per_agent": bpa, "GB/s": round(gbs, 1),
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": kern}
-    res["steer_for_clustering_ms"] = round(sum(res[k]["ms"] for k in ("steer_forces", "similarities",
-                                                                        "cluster_steer", "apply")), 5)
+    res["steer_for_clustering_ms"] = round(sum(res[k]["ms"] for k in ("steer_forces", "similarities_steer",
+                                                                        "apply")), 5)
     ctx.close()
     return {"workload": w.name, "n": n, "K": K, "features": features, "lp_iterations": lp_iterations,
             "reps": reps, "passes": res}
+
+
+def consumers(reps=20) -> dict:
+    """The small list consumers on the 1,048,576-boid blob flock (K 32), on the
+    lists and sorted state of one brt_step: steerAndSlowToAvoidNeighbors, the
+    neighbourhood colouring and the shape-target attraction, each repeated
+    `reps` times between HIP events on the launch stream."""
+    import numpy as np
+    import torch
+
+    from behavert_amd import workloads as W
+    from behavert_amd.engine import Context, Flock
+    from behavert_amd.types import SteerParams, list_stride
+
+    w = W.blobs_1m()
+    n, K = w.n, w.prox.max_neighbors
+    Wd = list_stride(K)
+    ctx = Context(0)
+    fl = Flock(ctx, w.grid, w.prox, w.steer, w.state)
+    fl.step(w.dt, 0)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(0)
+    targets = torch.from_numpy(np.c_[rng.uniform(0, 400, (n, 3)), np.ones(n)].astype(np.float32)).cuda()
+    f0 = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    f1, fwd = torch.empty_like(f0), torch.empty_like(f0)
+    color = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    sp = SteerParams.make()
+    passes = {
+        "slow_avoid": (lambda: ctx.steer_and_slow_to_avoid_neighbors(w.prox, sp, fl.sorted_pos, fl.sorted_forward,
+                                                                     fl.lists, fwd, f0, f1),
+                       32 + 4 * Wd + 16 + 32, "k_slow_avoid"),
+        "neighborhood_color": (lambda: ctx.neighborhood_color(w.prox, (0.1, 0.2, 0.3), 1, fl.hash, fl.lists, color),
+                               8 + 4 + 16, "k_neighborhood_color"),
+        "move_towards_target": (lambda: ctx.move_towards_target(1, (0.0, 0.0, 0.0), fl.hash, fl.sorted_pos, targets,
+                                                                f0, f1),
+                                8 + 16 + 16 + 16 + 16, "k_move_towards_target"),
+    }
+    res = {}
+    for name, (fn, bpa, kern) in passes.items():
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = bpa * n / 1e9 / (ms / 1e3)
+        res[name] = {"ms": round(ms, 5), "bytes_per_agent": bpa, "GB/s": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": kern}
+    ctx.close()
+    return {"workload": w.name, "n": n, "K": K, "reps": reps, "passes": res}
 
 
 def batched(steps=20, loop_steps=3) -> dict:
@@ -344,7 +401,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10, help="CPU baseline sample steps (0 disables)")
     ap.add_argument("--no-phases", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and batched measurements")
-    ap.add_argument("--only", choices=("adjoint", "batched", "full_stack", "clustering"),
+    ap.add_argument("--only", choices=("adjoint", "batched", "full_stack", "clustering", "consumers"),
                     help="run one extra section alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with three 256 MiB device copies (the reference kernel for counter calibration)")
@@ -356,7 +413,8 @@ def main():
 
     if args.only:
         print(json.dumps({args.only: {"adjoint": adjoint_ratio, "batched": batched,
-                                      "full_stack": full_stack, "clustering": clustering}[args.only]()}),
+                                      "full_stack": full_stack, "clustering": clustering,
+                                      "consumers": consumers}[args.only]()}),
               flush=True)
         return
 
@@ -439,6 +497,7 @@ def main():
         extras["batched"] = batched()
         extras["full_stack"] = full_stack()
         extras["clustering"] = clustering()
+        extras["consumers"] = consumers()
     cpu = cpu_baseline(w, args.cpu_steps) if args.cpu_steps > 0 else None
     line = {
         "metric": "agent-updates/s at 1M boids (% HBM peak); fwd+adjoint vs plain step time",

@@ -263,6 +263,40 @@ int brt_osw_apply_steering_force(brt_context* ctx, const brt_grid_params* grid,
                                  const float* force_in, float* pos_out, float* forward_out,
                                  float* smooth_accel_out, float* force_out, void* stream);
 
+/* OpenSteerWrapperPlugIn::steerAndSlowToAvoidNeighbors (OpenSteerWrapperPlugIn.h:309-325,
+ * kernel steerToAvoidNeighbors OpenSteerWrapper_kernel.cu:110-154), per sorted agent:
+ *   s = calcSeparation * separation weight; when the list holds c > 16 agents,
+ *   s *= (c - 16) * 0.5; speed *= 0.9 when dot(s, forward) < -0.8, *= 1.02 when
+ *   > 0.5 (double-precision constants, as compiled for sm_20+);
+ *   forward_out = (forward, speed) sorted; force_out = s + force_in (may alias). */
+int brt_osw_steer_and_slow_to_avoid_neighbors(brt_context* ctx, const brt_prox_params* prox,
+                                              const brt_steer_params* steer, uint32_t n,
+                                              const float* sorted_pos, const float* sorted_forward,
+                                              const uint32_t* lists, float* forward_out,
+                                              const float* force_in, float* force_out, void* stream);
+
+/* Drawable3DPlugIn's neighbourhood colouring (drawBodiesAsPointSprites with
+ * neighborhoodColoring, Drawable3DPlugIn.h:712-724; extractColorFromNeighborhoodD
+ * drawable3d_kernel.cu:60-93): f = list count / K,
+ *   color = (base.x + f, base.y + 0.3 + 1 - f, base.z + f (1 - f), 1)
+ * by agent ID; with vertexes_per_agent = 3 (useCUDAGeometry) three float4 per
+ * agent, the first minus (0.5, 0.5, 0.5, 0). */
+int brt_draw_neighborhood_color(brt_context* ctx, const brt_prox_params* prox,
+                                const float color_base[3], uint32_t vertexes_per_agent, uint32_t n,
+                                const uint32_t* hash, const uint32_t* lists, float* color,
+                                void* stream);
+
+/* Shapes3DPlugIn::moveTowardsTarget (Shapes3DPlugIn.h:552-580, moveTowardsTargetD
+ * Shapes3D_kernel.cu:63-87): t = targets[id + index_shift], the index reduced by
+ * n when it exceeds n (index n reads as (0,0,0,0), as the reference's texture
+ * fetch past the end does); force_out = normalize((t.xyz + target_base) - pos)
+ * * 12 * t.w + force_in (sorted, may alias).  targets: float4[n] by shifted ID;
+ * 0 <= index_shift < n (the host keeps it there). */
+int brt_shapes_move_towards_target(brt_context* ctx, int index_shift, const float target_base[3],
+                                   uint32_t n, const uint32_t* hash, const float* sorted_pos,
+                                   const float* targets, const float* force_in, float* force_out,
+                                   void* stream);
+
 /* ---------------------------------------------------------------- BehaviorClustering */
 
 /* BehaviorClusteringParams (PlugIns/BehaviorClustering/include/BehaviorClustering_kernel.cuh:40-59),
@@ -310,6 +344,18 @@ int brt_bc_steer_for_clustering(brt_context* ctx, const brt_prox_params* prox,
                                 uint32_t n, const float* sorted_pos, const float* sorted_forward,
                                 const uint32_t* lists, const float* similarities,
                                 const float* force_in, float* force_out, void* stream);
+
+/* BehaviorClusteringPlugIn::steerForClustering (BehaviorClusteringPlugIn.h:740-817)
+ * in one pass: brt_bc_neigh_similarities followed by brt_bc_steer_for_clustering,
+ * with the same outputs bit for bit (similarities, stats_out, force_out), the
+ * cluster sums accumulated as each similarity is produced instead of re-reading
+ * the lists and similarities.  force_in/force_out float4 sorted (may alias). */
+int brt_bc_similarities_steer(brt_context* ctx, const brt_prox_params* prox,
+                              const brt_steer_params* steer, const brt_bc_params* bc, uint32_t n,
+                              const uint32_t* hash, const uint32_t* lists, const float* features,
+                              const float* stats_in, float* stats_out, float* similarities,
+                              const float* sorted_pos, const float* sorted_forward,
+                              const float* force_in, float* force_out, void* stream);
 
 /* BehaviorClusteringPlugIn::computeClusterMembership (BehaviorClusteringPlugIn.h:823-865):
  * local propagation of cluster labels over the neighbour lists.

@@ -71,6 +71,9 @@ def lib():
             "orc_bc_cluster_steer_f32": (None, [PP, SP, C.POINTER(BcParams), U, P, P, P, P, P, P]),
             "orc_bc_cluster_steer_f64": (None, [PP, SP, C.POINTER(BcParams), U, P, P, P, P, P, P]),
             "orc_cluster_membership": (None, [PP, U, U, P, P, P, P, P, P]),
+            "orc_slow_avoid": (None, [PP, SP, U, P, P, P, P, P, P]),
+            "orc_neighborhood_color": (None, [PP, P, U, U, P, P, P]),
+            "orc_move_towards_target": (None, [C.c_int, P, U, P, P, P, P, P]),
             "orc_cluster_hit_rate": (None, [PP, F, U, P, P, P, P, P]),
             "orc_steer_forces_f32": (None, [PP, SP, F, U, U, P, P, P, P, P, P]),
             "orc_steer_forces_f64": (None, [PP, SP, F, U, U, P, P, P, P, P, P]),
@@ -302,6 +305,38 @@ def cluster_hit_rate(prox, radius, h, lists, spos, membership):
                                _p(np.ascontiguousarray(lists, np.uint32)), _p(np.ascontiguousarray(spos, np.float32)),
                                _p(np.ascontiguousarray(membership, np.float32)), _p(m_out))
     return m_out
+
+
+def slow_avoid(prox, steer, spos, sfwd, lists, force_in):
+    """steerToAvoidNeighbors: (forward_out sorted, force_out sorted)."""
+    n = len(spos)
+    fwd = np.zeros((n, 4), np.float32)
+    force = np.zeros((n, 4), np.float32)
+    lib().orc_slow_avoid(C.byref(prox), C.byref(steer), n, _p(np.ascontiguousarray(spos, np.float32)),
+                         _p(np.ascontiguousarray(sfwd, np.float32)), _p(np.ascontiguousarray(lists, np.uint32)),
+                         _p(fwd), _p(np.ascontiguousarray(force_in, np.float32)), _p(force))
+    return fwd, force
+
+
+def neighborhood_color(prox, base, vertexes, h, lists):
+    """extractColorFromNeighborhoodD: (n, vertexes, 4) colours by ID."""
+    n = len(h)
+    out = np.zeros((n, vertexes, 4), np.float32)
+    lib().orc_neighborhood_color(C.byref(prox), _p(np.asarray(base, np.float32)), int(vertexes), n,
+                                 _p(np.ascontiguousarray(h, np.uint32)), _p(np.ascontiguousarray(lists, np.uint32)),
+                                 _p(out))
+    return out
+
+
+def move_towards_target(shift, base, h, spos, targets, force_in):
+    """moveTowardsTargetD: the new force (n, 4) sorted."""
+    n = len(h)
+    out = np.zeros((n, 4), np.float32)
+    lib().orc_move_towards_target(int(shift), _p(np.asarray(base, np.float32)), n,
+                                  _p(np.ascontiguousarray(h, np.uint32)), _p(np.ascontiguousarray(spos, np.float32)),
+                                  _p(np.ascontiguousarray(targets, np.float32)),
+                                  _p(np.ascontiguousarray(force_in, np.float32)), _p(out))
+    return out
 
 
 class HostFlock:

@@ -1116,6 +1116,93 @@ void orc_cluster_hit_rate(const brt_prox_params* p, float radius, uint32_t n, co
     }
 }
 
+// steerToAvoidNeighbors (OpenSteerWrapper_kernel.cu:110-154): calcSeparation
+// (OpenSteerWrapper_resources.cu:166-211) times its weight, amplified above 16
+// listed agents, then the speed rule with the reference's double constants.
+void orc_slow_avoid(const brt_prox_params* p, const brt_steer_params* sp, uint32_t n, const float* spos,
+                    const float* sfwd, const uint32_t* lists, float* fwd_out, const float* force_in,
+                    float* force_out) {
+    set_threads();
+    const uint32_t W = listStride(p->max_neighbors);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const V3<float> myPos = ld3(spos, (uint32_t)i), myFwd = ld3(sfwd, (uint32_t)i);
+        const uint32_t* L = lists + (size_t)i * W;
+        const uint32_t count = L[0];
+        V3<float> steering = v3<float>(0, 0, 0);
+        uint32_t neighbors = 0;
+        for (uint32_t q = 0; q < count; ++q) {
+            const V3<float> other = ld3(spos, L[1 + q]);
+            if (inBoidNeighborhood(myPos, other, sp->sensing_min_radius, sp->separation[1], sp->separation[2],
+                                   myFwd)) {
+                const V3<float> offset = other - myPos;
+                const float d2 = dot(offset, offset);
+                steering = steering + offset / (-d2);
+                ++neighbors;
+            }
+        }
+        if (neighbors > 0) steering = normalize(steering / (float)neighbors);
+        steering = steering * sp->separation[0];
+        const int neighNumThreshold = 16;
+        const float steerForceMultipler = 0.5f;
+        if (count > (uint32_t)neighNumThreshold)
+            steering = steering * ((float)(count - neighNumThreshold) * steerForceMultipler);
+        float mySpeed = sfwd[4 * i + 3];
+        if (dot(steering, myFwd) < -0.8)
+            mySpeed *= 0.9;
+        else if (dot(steering, myFwd) > 0.5)
+            mySpeed *= 1.02;
+        fwd_out[4 * i] = myFwd.x;
+        fwd_out[4 * i + 1] = myFwd.y;
+        fwd_out[4 * i + 2] = myFwd.z;
+        fwd_out[4 * i + 3] = mySpeed;
+        force_out[4 * i] = steering.x + force_in[4 * i];
+        force_out[4 * i + 1] = steering.y + force_in[4 * i + 1];
+        force_out[4 * i + 2] = steering.z + force_in[4 * i + 2];
+        force_out[4 * i + 3] = 0.0f;
+    }
+}
+
+// extractColorFromNeighborhoodD (drawable3d_kernel.cu:60-93), by agent ID.
+void orc_neighborhood_color(const brt_prox_params* p, const float* base, uint32_t vertexes, uint32_t n,
+                            const uint32_t* hash, const uint32_t* lists, float* color) {
+    const uint32_t W = listStride(p->max_neighbors);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t id = hash[2 * i + 1];
+        const float colorFactor = ((float)lists[(size_t)i * W]) / (float)p->max_neighbors;
+        const float c[4] = {base[0] + colorFactor, (float)(base[1] + 0.3 + 1 - colorFactor),
+                            base[2] + colorFactor * (1 - colorFactor), 1.0f};
+        if (vertexes == 3) {
+            for (int k = 0; k < 4; ++k) {
+                color[12 * (size_t)id + k] = c[k] - (k < 3 ? 0.5f : 0.0f);
+                color[12 * (size_t)id + 4 + k] = c[k];
+                color[12 * (size_t)id + 8 + k] = c[k];
+            }
+        } else {
+            for (int k = 0; k < 4; ++k) color[4 * (size_t)id + k] = c[k];
+        }
+    }
+}
+
+// moveTowardsTargetD (Shapes3D_kernel.cu:63-87); index n (and beyond) reads as zeros.
+void orc_move_towards_target(int shift, const float* base, uint32_t n, const uint32_t* hash, const float* spos,
+                             const float* targets, const float* force_in, float* force_out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        int shiftedIndex = (int)hash[2 * i + 1] + shift;
+        if (shiftedIndex > (int)n) shiftedIndex -= (int)n;
+        float t[4] = {0, 0, 0, 0};
+        if (shiftedIndex >= 0 && shiftedIndex < (int)n)
+            for (int k = 0; k < 4; ++k) t[k] = targets[4 * (size_t)shiftedIndex + k];
+        const V3<float> steering = (v3<float>(t[0], t[1], t[2]) + v3<float>(base[0], base[1], base[2])) -
+                                   ld3(spos, i);
+        const V3<float> f = (normalize(steering) * 12.0f) * t[3];
+        force_out[4 * i] = f.x + force_in[4 * i];
+        force_out[4 * i + 1] = f.y + force_in[4 * i + 1];
+        force_out[4 * i + 2] = f.z + force_in[4 * i + 2];
+        force_out[4 * i + 3] = 0.0f;
+    }
+}
+
 // Whole FP32 step on host buffers laid out as brt_step_buffers (host pointers).
 // Phase wall times (seconds) are added into phase_s[0..4] when non-null.
 void orc_step_f32(const brt_grid_params* g, const brt_prox_params* p, const brt_steer_params* s,

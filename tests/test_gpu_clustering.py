@@ -90,6 +90,33 @@ def test_similarities_bit_exact(ctx, orc, F, adaptive):
     assert o["lists"][:, 0].max() > 0
 
 
+@pytest.mark.parametrize("K", [7, 31, 64, 100])
+def test_similarities_list_widths(ctx, orc, K):
+    """List widths on both sides of the LDS row-tile switch (256 agents up to
+    K = 39, one wave above) and with an even row length (K + 1 = 32); the
+    one-pass similarities + cluster steering as well."""
+    w = mixed_workload(n=20000, K=K, seed=K)
+    o = oracle_grid_outputs(orc, w.grid, w.prox, w.state)
+    F = 5
+    feats, stats = features_and_stats(w.n, F, seed=K)
+    bc = BcParams.make(F, adaptive=True)
+    sims, st = gpu_similarities(ctx, w.prox, bc, o, feats, stats)
+    rs, rst = orc.bc_similarities(w.prox, bc, o["hash"], o["lists"], feats, stats)
+    assert first_mismatch(sims.view(np.uint32), rs.view(np.uint32)) is None
+    assert first_mismatch(st.view(np.uint32), rst.view(np.uint32)) is None
+    sp = SteerParams.make()
+    f0 = np.random.default_rng(1).standard_normal((w.n, 4)).astype(np.float32)
+    s2 = torch.full((w.n, K + 1), float("nan"), dtype=torch.float32, device="cuda")
+    st2 = empty_f32(w.n, 4)
+    f1 = empty_f32(w.n, 4)
+    ctx.bc_similarities_steer(w.prox, sp, bc, dev(o["hash"].view(np.int32)), dev(o["lists"].view(np.int32)),
+                              dev(feats), dev(stats), st2, s2, dev(o["sorted_pos"]), dev(o["sorted_forward"]), dev(f0),
+                              f1)
+    rf = orc.bc_cluster_steer(w.prox, sp, bc, o["sorted_pos"], o["sorted_forward"], o["lists"], rs, f0)
+    assert first_mismatch(f32(s2).view(np.uint32), rs.view(np.uint32)) is None
+    assert first_mismatch(f32(f1).view(np.uint32), rf.view(np.uint32)) is None
+
+
 @pytest.mark.parametrize("variant", range(4))
 def test_decomposed_steering_equals_fused(ctx, orc, variant):
     """steer_forces -> apply_steering_force reproduces the fused steering pass bit
@@ -210,6 +237,14 @@ def test_clustering_step_bit_exact_and_fp64(ctx, orc, adaptive, record_property)
     ctx.bc_similarities(w.prox, bc, h, L, dev(feats), dev(stats), st_out, sims)
     f1 = empty_f32(n, 4)
     ctx.bc_steer_for_clustering(w.prox, sp, bc, *args, L, sims, f0, f1)
+    # the one-pass form (similarities + cluster steering) gives the same bits
+    sims_b = torch.full_like(sims, float("nan"))
+    st_b = torch.full_like(st_out, float("nan"))
+    f1_b = torch.full_like(f1, float("nan"))
+    ctx.bc_similarities_steer(w.prox, sp, bc, h, L, dev(feats), dev(stats), st_b, sims_b, *args, f0, f1_b)
+    for a_, b_, k in ((sims, sims_b, "similarities"), (f1, f1_b, "force")) + (((st_out, st_b, "stats"),)
+                                                                               if adaptive else ()):
+        assert first_mismatch(f32(a_).view(np.uint32), f32(b_).view(np.uint32)) is None, k
     out = [empty_f32(n, 4) for _ in range(4)]
     ctx.apply_steering_force(w.grid, sp, w.dt, *args, dev(o["sorted_smooth_accel"]), h, f1, *out[:3],
                              force_out=out[3])

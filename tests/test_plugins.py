@@ -78,8 +78,9 @@ def test_plugin_chain_matches_oracle(tmp_path, orc):
 def test_clustering_plugin_chain_matches_oracle(tmp_path, orc):
     """BehaveRT_Clustering's update sequence through the plug-in API
     (BehaviorClusteringPlugIn::steerForClustering between the flocking
-    behaviours and the world-centre seek) against the oracle's passes run in
-    the same order, bit for bit after 5 steps."""
+    behaviours and the world-centre seek, computeClusterMembership after it)
+    against the oracle's passes run in the same order, bit for bit after 5
+    steps."""
     from behavert_amd.types import BcParams, SteerParams
 
     from ._parity import oracle_grid_outputs
@@ -101,7 +102,9 @@ def test_clustering_plugin_chain_matches_oracle(tmp_path, orc):
     r = subprocess.run([B.CLUSTERING_BIN, cfg, str(steps), str(init), str(out)], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr
-    got = np.fromfile(out, np.float32).reshape(5, w.n, 4)
+    raw = np.fromfile(out, np.float32)
+    got = raw[:5 * w.n * 4].reshape(5, w.n, 4)
+    memb = raw[5 * w.n * 4:].reshape(w.n, 3)
 
     sp = SteerParams.make(weights=(1.0, 0.8, 0.6), seek_world_center=True)
     bc = BcParams.make(F, adaptive=True, stats_params=(0.1, 0.1, 0.001), cluster_weight=0.9, alignment_weight=0.7)
@@ -111,9 +114,11 @@ def test_clustering_plugin_chain_matches_oracle(tmp_path, orc):
         f0, _ = orc.steer_forces(w.prox, sp, w.dt, 0, o["sorted_pos"], o["sorted_forward"], o["hash"], o["lists"])
         sims, stats = orc.bc_similarities(w.prox, bc, o["hash"], o["lists"], feats, stats)
         f1 = orc.bc_cluster_steer(w.prox, sp, bc, o["sorted_pos"], o["sorted_forward"], o["lists"], sims, f0)
+        m, _ = orc.cluster_membership(w.prox, 10, o["hash"], o["lists"], o["sorted_pos"])
         res = orc.apply_steering_force(w.grid, sp, w.dt, o["sorted_pos"], o["sorted_forward"],
                                        o["sorted_smooth_accel"], o["hash"], f1)
         st = {k: res[k] for k in ("pos", "forward", "smooth_accel")}
     ref = (st["pos"], st["forward"], st["smooth_accel"], stats, res["force"])
     for a, b, k in zip(got, ref, ("pos", "forward", "smooth_accel", "stats", "force")):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k
+    assert np.array_equal(memb.view(np.uint32), m.view(np.uint32)), "cluster membership"
