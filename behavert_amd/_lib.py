@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .types import BatchConfig, BcParams, FlockDesc, GridParams, ProxParams, SteerParams, StepBuffers
+from .types import BatchConfig, BcParams, FlockDesc, GridParams, ProxParams, SnapshotInfo, SteerParams, StepBuffers
 
 # BRT_LIB names another build of the same library (e.g. one compiled with
 # different launch constants); the default is the in-tree build.
@@ -52,6 +52,9 @@ SIGNATURES = {
     "brt_bc_similarities_steer": (_I, [_P, _PP, _SP, C.POINTER(BcParams), _U32, _P, _P, _P, _P, _P, _P, _P, _P,
                                        _P, _P, _P]),
     "brt_bc_cluster_membership": (_I, [_P, _PP, _U32, _U32, _P, _P, _P, _P, _P, _P, _P]),
+    "brt_snapshot_save": (_I, [_P, C.c_char_p, _GP, _PP, _U32, _U32, C.POINTER(StepBuffers), _P]),
+    "brt_snapshot_read_info": (_I, [C.c_char_p, C.POINTER(SnapshotInfo)]),
+    "brt_snapshot_load": (_I, [_P, C.c_char_p, _U32, C.POINTER(StepBuffers), C.POINTER(SnapshotInfo), _P]),
     "brt_osw_steer_and_slow_to_avoid_neighbors": (_I, [_P, _PP, _SP, _U32, _P, _P, _P, _P, _P, _P, _P]),
     "brt_draw_neighborhood_color": (_I, [_P, _PP, C.POINTER(_F), _U32, _U32, _P, _P, _P, _P]),
     "brt_shapes_move_towards_target": (_I, [_P, _I, C.POINTER(_F), _U32, _P, _P, _P, _P, _P, _P]),

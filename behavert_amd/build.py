@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbehavert_amd.so")
 SOURCES = ["brt_envgrid.hip", "brt_proximity.hip", "brt_opensteer.hip", "brt_clustering.hip", "brt_consumers.hip",
-           "brt_batch.hip", "brt_adjoint.hip", "brt_capi.hip"]
+           "brt_batch.hip", "brt_adjoint.hip", "brt_snapshot.hip", "brt_capi.hip"]
 ARCH = os.environ.get("BRT_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -9,17 +9,19 @@ BehaveRT_Flocking.cpp:177-196):
                                  applySteeringForces (fused)
   Flock.update_features(rate)    BehaviorClustering feature update
   Flock.step(dt, i)              the whole step in one C call (brt_step)
+  Flock.save(path) / load(path)  binary state snapshots (brt_snapshot_*)
 """
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 import torch
 
 from ._lib import BrtError, lib
-from .types import (NUM_PHASES, PHASE_NAMES, BatchConfig, FlockDesc, GridParams, ProxParams, SteerParams,
-                    StepBuffers, list_stride)
+from .types import (NUM_PHASES, PHASE_NAMES, SNAP_FEATURES, SNAP_LISTS, SNAP_WANDER, BatchConfig, FlockDesc,
+                    GridParams, ProxParams, SnapshotInfo, SteerParams, StepBuffers, list_stride)
 
 
 def _ptr(t: torch.Tensor | None) -> int | None:
@@ -284,6 +286,27 @@ class Flock:
                                               C.byref(b), current_stream()))
         if self.features is not None and feature_rate != 0.0:
             self.features, self.features_out = self.features_out, self.features
+
+    def save(self, path: str, step: int, flags: int | None = None):
+        """Binary snapshot of the flock (brt_snapshot_save): the state by ID, the
+        last step's hash and lists, and the features and wander state when the
+        flock has them (or the BRT_SNAP_* flags given)."""
+        if flags is None:
+            flags = SNAP_LISTS | (SNAP_FEATURES if self.features is not None else 0) | \
+                (SNAP_WANDER if self.wander is not None else 0)
+        b = self.buffers()
+        self.ctx.check(self.ctx._lib.brt_snapshot_save(self.ctx.handle, os.fsencode(path), C.byref(self.grid),
+                                                       C.byref(self.prox), int(step), int(flags), C.byref(b),
+                                                       current_stream()))
+
+    def load(self, path: str, flags: int = 0) -> SnapshotInfo:
+        """Restores the state (and the sections in `flags`) from a snapshot of a
+        flock of the same size; returns its header (the step it was saved at)."""
+        info = SnapshotInfo()
+        b = self.buffers()
+        self.ctx.check(self.ctx._lib.brt_snapshot_load(self.ctx.handle, os.fsencode(path), int(flags), C.byref(b),
+                                                       C.byref(info), current_stream()))
+        return info
 
     def state(self) -> dict:
         torch.cuda.synchronize()

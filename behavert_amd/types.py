@@ -161,6 +161,26 @@ class StepBuffers(C.Structure):
 
 MAX_FEATURES = 32
 
+SNAP_LISTS = 1
+SNAP_FEATURES = 2
+SNAP_WANDER = 4
+
+
+class SnapshotInfo(C.Structure):
+    """brt_snapshot_info: the header of a state snapshot."""
+    _fields_ = [
+        ("version", C.c_uint32),
+        ("n", C.c_uint32),
+        ("max_neighbors", C.c_uint32),
+        ("list_stride", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("step", C.c_uint32),
+        ("grid", GridParams),
+        ("prox", ProxParams),
+        ("payload_bytes", C.c_uint64),
+        ("checksum", C.c_uint64),
+    ]
+
 
 class BcParams(C.Structure):
     """BehaviorClusteringParams subset read by the similarity and cluster-steering

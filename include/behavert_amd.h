@@ -427,6 +427,35 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid,
              float feature_rate, float dt, uint32_t step,
              const brt_step_buffers* buf, void* stream);
 
+/* ---------------------------------------------------------------- state snapshots
+ * Binary snapshot of a flock (no counterpart in the reference, which keeps no
+ * state on disk; its host round trips are DeviceArrayWrapper::copyArrayFromDevice
+ * / copyArrayToDevice, Core/include/DeviceArrayWrapper.h:345-386): a 256-byte
+ * header (magic "BRTSNAP\0", version 1, n, K, W, flags, step, grid and proximity
+ * parameters, payload size, FNV-1a 64 checksum) then pos, forward and
+ * smooth_accel (float4 by ID), with BRT_SNAP_LISTS the sorted hash and the
+ * lists, with BRT_SNAP_FEATURES the float4 features, with BRT_SNAP_WANDER the
+ * wander state.  Save copies device -> host on `stream` and synchronizes it;
+ * load checks magic, version, n, flags and checksum, then copies host -> device
+ * the sections asked for (flags must be a subset of the file's). */
+#define BRT_SNAP_LISTS 1u
+#define BRT_SNAP_FEATURES 2u
+#define BRT_SNAP_WANDER 4u
+typedef struct brt_snapshot_info {
+    uint32_t version, n, max_neighbors, list_stride, flags, step;
+    brt_grid_params grid;
+    brt_prox_params prox;
+    uint64_t payload_bytes;
+    uint64_t checksum;
+} brt_snapshot_info;
+int brt_snapshot_save(brt_context* ctx, const char* path, const brt_grid_params* grid,
+                      const brt_prox_params* prox, uint32_t step, uint32_t flags,
+                      const brt_step_buffers* buf, void* stream);
+/* Header only; needs no device. */
+int brt_snapshot_read_info(const char* path, brt_snapshot_info* info);
+int brt_snapshot_load(brt_context* ctx, const char* path, uint32_t flags,
+                      const brt_step_buffers* buf, brt_snapshot_info* info, void* stream);
+
 /* Per-phase timing of brt_step with hipEvents recorded on the step's stream
  * (the reference harness's per-phase timers, BehaveRT_Flocking.cpp:171-204).
  * Phases: 0 hash+sort, 1 reorder+ranges, 2 neighbours, 3 steering+integration,

@@ -17,6 +17,15 @@ restates.  Parity pins (tests/test_oracle_*.py):
   bc_similarities .. closed forms (parallel / opposite / orthogonal features),
                      hand-traced statistics updates, interpolation end points
   bc_cluster_steer . two-agent KATs (similarity 1 -> seek, 0 -> flee)
+  steer_forces, apply_steering_force
+                     bit-equal, together, to the fused steer_f32/f64
+  cluster_membership labels = least sorted index within t list hops (breadth-
+                     first sets); connected components (scipy) at convergence
+  cluster_hit_rate . KAT at distances 1.0 / 1.5 (on the radius) / 1.6
+  slow_avoid, neighborhood_color, move_towards_target
+                     hand-derived KATs (tests/test_oracle_consumers.py)
+  snapshot.py ...... numpy restatement of the snapshot format; FNV-1a 64 test
+                     vectors (tests/test_snapshot.py)
 """
 from __future__ import annotations
 
