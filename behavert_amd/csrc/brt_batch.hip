@@ -50,7 +50,7 @@ struct brt_batch {
 namespace {
 
 uint32_t default_block(uint32_t K) {
-    const uint32_t W = brt::list_stride(K);
+    const uint32_t W = brt::lds_list_stride(K);
     uint32_t B = 256;
     while (B > 64 && (size_t)B * W * 4u > 40u * 1024u) B /= 2;
     return B;
@@ -60,8 +60,9 @@ int check_config(brt_context* ctx, const brt_batch* b, const brt_batch_config* c
     BRT_REQUIRE(c != nullptr, "config is NULL");
     BRT_REQUIRE(c->prox_block == 64 || c->prox_block == 128 || c->prox_block == 256,
                 "prox_block must be 64, 128 or 256");
-    BRT_REQUIRE(!c->lds_lists || (size_t)c->prox_block * brt::list_stride(b->prox.max_neighbors) * 4u <= 64u * 1024u,
-                "prox_block * list stride exceeds 64 KB of LDS");
+    BRT_REQUIRE(!c->lds_lists ||
+                    (size_t)c->prox_block * brt::lds_list_stride(b->prox.max_neighbors) * 4u <= 64u * 1024u,
+                "prox_block * LDS list stride exceeds 64 KB of LDS");
     BRT_REQUIRE(c->uniform_max_cells <= 8, "uniform_max_cells must be in 0..8");
     BRT_REQUIRE(c->sort_items == 8 || c->sort_items == 16, "sort_items must be 8 or 16");
     return BRT_OK;

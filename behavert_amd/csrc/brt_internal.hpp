@@ -114,6 +114,14 @@ hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_
                           hipStream_t s);
 
 inline uint32_t list_stride(uint32_t K) { return 4u * (K / 4u + 1u); }
+// Row stride of the LDS list tile of the neighbour kernels: the list stride,
+// widened so a round of 8 candidates can append past K without a per-candidate
+// count check (rows of W >= K + 8 words, a multiple of 4), unless a 64-lane
+// tile of such rows would exceed 64 KB (then the rounds check the count).
+inline uint32_t lds_list_stride(uint32_t K) {
+    const uint32_t W = list_stride(K), need = (K + 8u + 3u) / 4u * 4u;
+    return (W >= need || need > 256u) ? W : need;
+}
 
 // ---------------------------------------------------------------- batches
 // One flock of a batch as the kernels see it.  Flocks are concatenated: flock f

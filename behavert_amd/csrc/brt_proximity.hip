@@ -35,19 +35,32 @@
 #include "brt_steer_common.hpp"
 
 // Candidates tested per round of the scan loops (their loads are in flight
-// together): per-lane (vector) rounds and wave-uniform (scalar) rounds.
+// together): per-lane (vector) rounds with LDS lists and with register-chunk
+// lists, and wave-uniform (scalar) rounds.  Measured on gfx950: 8 per round
+// with LDS lists (K = 32 blobs: -11 % against 4), 4 with register-chunk lists
+// (8 spills there).
 #ifndef BRT_PROX_GROUP
-#define BRT_PROX_GROUP 4
+#define BRT_PROX_GROUP 8
+#endif
+#ifndef BRT_PROX_GROUP_R
+#define BRT_PROX_GROUP_R 4
 #endif
 #ifndef BRT_PROX_GROUP_U
 #define BRT_PROX_GROUP_U 4
+#endif
+// 1: with LDS lists, cells that cannot hold the agent itself run rounds
+// without the self test (a second copy of the round code; with register-chunk
+// lists the larger code measured slower); 0: every round tests for the agent.
+#ifndef BRT_PROX_SELF_SPLIT
+#define BRT_PROX_SELF_SPLIT 1
 #endif
 
 namespace brt {
 
 namespace {
 
-constexpr int kGroup = BRT_PROX_GROUP;
+constexpr int kGroupL = BRT_PROX_GROUP;
+constexpr int kGroupR = BRT_PROX_GROUP_R;
 constexpr int kGroupU = BRT_PROX_GROUP_U;
 
 // Loads through the constant address space: with a wave-uniform index these
@@ -65,6 +78,7 @@ __device__ __forceinline__ float4 cload(cfloat4_ptr p, uint32_t i) {
 struct ProxArgs {
     brt_grid_params g;  // single-grid launches
     uint32_t K, W, n;   // n = all agents of the launch (a batch: of every flock)
+    uint32_t Ws;        // row stride of the LDS list tile (lds_list_stride(K) >= W)
     float T;            // exact squared-radius threshold
     float Tskip;        // T * (1 + 1e-5): cell-skip threshold on the box distance
     float slack[3];     // per-axis widening of cell boxes for rounding (single grid)
@@ -131,6 +145,7 @@ struct Lane {
     uint32_t i;
     uint32_t cnt;
     uint32_t* row;       // the agent's list row: staged in LDS, or in global memory
+    uint32_t* wp;        // LDS lists: the next free entry (row + 1 + cnt) during a round
     uint32_t h0, h1, h2;  // register-chunk lists: entries 0..2 (chunk 0, with the count)
     uint32_t b0, b1, b2;  // register-chunk lists: entries of the pending 16-byte chunk
     // squared distances from the agent to the boxes of the cells one below
@@ -213,32 +228,52 @@ __device__ __forceinline__ void finish_list(Lane& L, uint32_t W) {
     row[0] = make_uint4(k, k > 0u ? L.h0 : P, k > 1u ? L.h1 : P, k > 2u ? L.h2 : P);
 }
 
-// Tests the candidates j .. j+kGroup-1 (positions q) for lane L and appends
+// Tests the candidates j .. j+G-1 (positions q) for lane L and appends
 // those that pass, in order, without branching.  rem = lim - j (candidates
 // left in the cell), di = i - j (where the agent itself sits in the round).
-// CAP: some lane may reach K inside this round, so every append re-checks
-// the count.
-template <bool LDSL, bool CAP, int G>
+// SELF: the agent's own index lies in this cell's range, so it is excluded.
+// CAP (some lane may reach K inside this round): 1 = every append re-checks
+// the count; 2 = the LDS row has room for G - 1 entries past K (stride
+// Ws >= K + G, every lane entering the round below K: the last write lands at
+// most at row[K + G - 1]), so the round appends freely and the count is
+// clamped to K afterwards: entries past K are overwritten by padding.
+// LDS lists append through a write pointer: a predicated store and a pointer
+// bump per candidate.
+template <bool LDSL, int CAP, bool SELF, int G>
 __device__ __forceinline__ void test_round(const ProxArgs& a, Lane& L, uint32_t j, uint32_t rem, uint32_t di,
                                            const float4* q) {
+    if (LDSL) L.wp = L.row + 1u + L.cnt;
 #pragma unroll
     for (int t = 0; t < G; ++t) {
         const float rx = q[t].x - L.px, ry = q[t].y - L.py, rz = q[t].z - L.pz;
         const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
         const float d2 = (xx + yy) + zz;
-        bool take = ((uint32_t)t < rem) & (di != (uint32_t)t) & !(d2 > a.T);
-        if (CAP) take = take & (L.cnt < a.K);
-        append<LDSL>(L, j + (uint32_t)t, take);
+        bool take = ((uint32_t)t < rem) & !(d2 > a.T);
+        if (SELF) take = take & (di != (uint32_t)t);
+        if (LDSL) {
+            if (CAP == 1) take = take & (L.wp < L.row + 1u + a.K);
+            if (take) *L.wp = j + (uint32_t)t;
+            L.wp += take ? 1 : 0;
+        } else {
+            if (CAP) take = take & (L.cnt < a.K);
+            append<false>(L, j + (uint32_t)t, take);
+        }
+    }
+    if (LDSL) {
+        const uint32_t c = (uint32_t)(L.wp - (L.row + 1u));
+        L.cnt = CAP == 2 ? (c < a.K ? c : a.K) : c;
     }
 }
 
-template <bool LDSL, int G>
+template <bool LDSL, bool SELF, int G>
 __device__ __forceinline__ void test_round_any(const ProxArgs& a, Lane& L, uint32_t j, uint32_t rem, uint32_t di,
                                                const float4* q) {
     if (__ballot(L.cnt + (uint32_t)G > a.K) == 0)
-        test_round<LDSL, false, G>(a, L, j, rem, di, q);
+        test_round<LDSL, 0, SELF, G>(a, L, j, rem, di, q);
+    else if (LDSL && a.Ws >= a.K + (uint32_t)G)
+        test_round<LDSL, 2, SELF, G>(a, L, j, rem, di, q);
     else
-        test_round<LDSL, true, G>(a, L, j, rem, di, q);
+        test_round<LDSL, 1, SELF, G>(a, L, j, rem, di, q);
 }
 
 // Buffer resource over a float4 array of n elements: 32-bit offsets with an
@@ -280,11 +315,18 @@ __device__ __forceinline__ void scan_uniform(const ProxArgs& a, const Seg& sg, L
                     const bool need = (L.cnt < a.K) & !(bd > a.Tskip);
                     if (__ballot(need) == 0) continue;
                 }
+                const bool self_here = s <= L.i && L.i < lim;
+                const bool any_self = !(LDSL && BRT_PROX_SELF_SPLIT) || __ballot(self_here) != 0;
                 for (uint32_t j = s; j < lim; j += kGroupU) {
                     float4 q[kGroupU];
 #pragma unroll
                     for (int t = 0; t < kGroupU; ++t) q[t] = cload(cpos, (j + t < lim) ? j + t : lim - 1);
-                    test_round_any<LDSL, kGroupU>(a, L, j, lim - j, L.i - j, q);
+                    if (L.cnt < a.K) {
+                        if (any_self)
+                            test_round_any<LDSL, true, kGroupU>(a, L, j, lim - j, L.i - j, q);
+                        else
+                            test_round_any<LDSL, false, kGroupU>(a, L, j, lim - j, L.i - j, q);
+                    }
                     if (__ballot(L.cnt < a.K) == 0) return;
                 }
             }
@@ -323,12 +365,18 @@ __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, La
             const uint32_t e = en[k];
             uint32_t lim = (e - s) > maxBodies ? s + maxBodies : e;
             lim = lim < a.n ? lim : a.n;
-            for (uint32_t j = s; j < lim; j += kGroup) {
+            // the agent's own index is in this range only in its own (stored) cell
+            const bool any_self = !(LDSL && BRT_PROX_SELF_SPLIT) || __ballot(s <= L.i && L.i < lim) != 0;
+            constexpr int G = LDSL ? kGroupL : kGroupR;
+            for (uint32_t j = s; j < lim; j += G) {
                 // candidates past lim are read (zeros past n) and not taken
-                float4 q[kGroup];
+                float4 q[G];
 #pragma unroll
-                for (int t = 0; t < kGroup; ++t) q[t] = bload(rpos, j * 16u + 16u * (uint32_t)t);
-                test_round_any<LDSL, kGroup>(a, L, j, lim - j, L.i - j, q);
+                for (int t = 0; t < G; ++t) q[t] = bload(rpos, j * 16u + 16u * (uint32_t)t);
+                if (any_self)
+                    test_round_any<LDSL, true, G>(a, L, j, lim - j, L.i - j, q);
+                else
+                    test_round_any<LDSL, false, G>(a, L, j, lim - j, L.i - j, q);
                 if (L.cnt >= a.K) return;
             }
         }
@@ -349,7 +397,7 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
     L.i = sg.row0 + lane;
     L.cnt = 0;
     const bool alive = wave_live && L.i < sg.end;
-    L.row = LDSL ? s_list + tid * a.W : a.lists + (size_t)(alive ? L.i : 0u) * a.W;
+    L.row = LDSL ? s_list + tid * a.Ws : a.lists + (size_t)(alive ? L.i : 0u) * a.W;
     L.h0 = L.h1 = L.h2 = L.b0 = L.b1 = L.b2 = BRT_LIST_PAD;
     float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
     if (alive) P = a.spos[L.i];
@@ -391,10 +439,14 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
     // LDS lists: each wave writes its own rows, 16 bytes per lane per store
     if (LDSL && wave_live) {
         const uint32_t rows = (sg.end - sg.row0) < 64u ? (sg.end - sg.row0) : 64u;
-        const uint32_t words4 = rows * a.W / 4u;
+        const uint32_t W4 = a.W / 4u, Ws4 = a.Ws / 4u;
+        const uint32_t words4 = rows * W4;
         uint4* dst = reinterpret_cast<uint4*>(a.lists + (size_t)sg.row0 * a.W);
-        const uint4* src = reinterpret_cast<const uint4*>(s_list + (size_t)(wid * 64u) * a.W);
-        for (uint32_t w = lane; w < words4; w += 64u) dst[w] = src[w];
+        const uint4* src = reinterpret_cast<const uint4*>(s_list + (size_t)(wid * 64u) * a.Ws);
+        for (uint32_t w = lane; w < words4; w += 64u) {
+            const uint32_t r = w / W4;
+            dst[w] = src[r * Ws4 + (w - r * W4)];
+        }
     }
 
     // fused steering: walk the finished list (from LDS, or the row this lane
@@ -520,6 +572,7 @@ ProxArgs make_args(const brt_grid_params& g, const brt_prox_params& p, float thr
     a.g = g;
     a.K = p.max_neighbors;
     a.W = list_stride(p.max_neighbors);
+    a.Ws = lds_list_stride(p.max_neighbors);
     a.n = n;
     a.T = threshold;
     // the skip threshold errs toward scanning: 1e-5 relative on d2 and a
@@ -554,9 +607,9 @@ hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, 
         return hipGetLastError();
     }
     ProxArgs a = make_args(g, p, threshold, spos, hash, cell_start, cell_end, n, lists, uniform_max_cells, xcd_remap);
-    const uint32_t B = prox_block(W);
+    const uint32_t B = prox_block(a.Ws);
     if (lds_lists)
-        hipLaunchKernelGGL((k_prox<false, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * W * 4u, s, a);
+        hipLaunchKernelGGL((k_prox<false, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.Ws * 4u, s, a);
     else
         hipLaunchKernelGGL((k_prox<false, false, false>), dim3((n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
@@ -575,9 +628,9 @@ hipError_t launch_neighbors_steer(const brt_grid_params& g, const brt_prox_param
     a.dt = dt;
     a.step = step;
     a.o = AgentOut{pos_out, fwd_out, sa_out, nullptr, wander};
-    const uint32_t B = prox_block(a.W);
+    const uint32_t B = prox_block(a.Ws);
     if (lds_lists)
-        hipLaunchKernelGGL((k_prox<true, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.W * 4u, s, a);
+        hipLaunchKernelGGL((k_prox<true, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.Ws * 4u, s, a);
     else
         hipLaunchKernelGGL((k_prox<true, false, false>), dim3((n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
@@ -598,10 +651,10 @@ hipError_t launch_batch_neighbors_steer(const BatchView& b, uint32_t n, const br
     a.step = 0;
     a.o = AgentOut{pos_out, fwd_out, sa_out, nullptr, wander};
     a.b = b;
-    const uint32_t B = block ? block : prox_block(a.W);
+    const uint32_t B = block ? block : prox_block(a.Ws);
     const uint32_t wpb = B / 64u;
     if (lds_lists)
-        hipLaunchKernelGGL((k_prox<true, true, true>), dim3((b.waves + wpb - 1) / wpb), dim3(B), (size_t)B * a.W * 4u,
+        hipLaunchKernelGGL((k_prox<true, true, true>), dim3((b.waves + wpb - 1) / wpb), dim3(B), (size_t)B * a.Ws * 4u,
                            s, a);
     else
         hipLaunchKernelGGL((k_prox<true, true, false>), dim3((b.waves + wpb - 1) / wpb), dim3(B), 0, s, a);
