@@ -35,10 +35,11 @@
 #include "brt_steer_common.hpp"
 
 // Candidates tested per round of the scan loops (their loads are in flight
-// together): per-lane (vector) rounds with LDS lists and with register-chunk
-// lists, and wave-uniform (scalar) rounds.  Measured on gfx950: 8 per round
-// with LDS lists (K = 32 blobs: -11 % against 4), 4 with register-chunk lists
-// (8 spills there).
+// together): per-lane (vector) rounds of dense waves with LDS lists, of sparse
+// waves and register-chunk lists, and wave-uniform (scalar) rounds.  Measured
+// on gfx950: 8 per round for dense waves with LDS lists (1M-boid blobs, K 32:
+// -11 % against 4), 4 for sparse waves (65k boids at ~1 per cell: 8 is 26 %
+// slower) and with register-chunk lists (8 spills there).
 #ifndef BRT_PROX_GROUP
 #define BRT_PROX_GROUP 8
 #endif
@@ -48,9 +49,9 @@
 #ifndef BRT_PROX_GROUP_U
 #define BRT_PROX_GROUP_U 4
 #endif
-// 1: with LDS lists, cells that cannot hold the agent itself run rounds
-// without the self test (a second copy of the round code; with register-chunk
-// lists the larger code measured slower); 0: every round tests for the agent.
+// 1: in dense waves with LDS lists, cells that cannot hold the agent itself run
+// rounds without the self test (a second copy of the round code, one ballot
+// per cell; elsewhere that measured slower); 0: every round tests for the agent.
 #ifndef BRT_PROX_SELF_SPLIT
 #define BRT_PROX_SELF_SPLIT 1
 #endif
@@ -337,7 +338,7 @@ __device__ __forceinline__ void scan_uniform(const ProxArgs& a, const Seg& sg, L
 // Vector mode: each lane walks its own canonical stream.  The nine cell ranges
 // of one x-offset slab are loaded together (one memory latency per slab rather
 // than two per cell).
-template <bool LDSL>
+template <bool LDSL, int G, bool SPLIT>
 __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, Lane& L, I3 gp) {
     const brt_grid_params& g = sg.g;
     const uint32_t maxBodies = g.max_bodies_per_cell;
@@ -366,8 +367,7 @@ __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, La
             uint32_t lim = (e - s) > maxBodies ? s + maxBodies : e;
             lim = lim < a.n ? lim : a.n;
             // the agent's own index is in this range only in its own (stored) cell
-            const bool any_self = !(LDSL && BRT_PROX_SELF_SPLIT) || __ballot(s <= L.i && L.i < lim) != 0;
-            constexpr int G = LDSL ? kGroupL : kGroupR;
+            const bool any_self = !SPLIT || __ballot(s <= L.i && L.i < lim) != 0;
             for (uint32_t j = s; j < lim; j += G) {
                 // candidates past lim are read (zeros past n) and not taken
                 float4 q[G];
@@ -430,8 +430,26 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
             pending &= ~__ballot(mine);
             if (mine) scan_uniform<LDSL>(a, sg, L, ux, uy, uz);
         }
-    } else if (alive) {
-        scan_vector<LDSL>(a, sg, L, gp);
+    } else {
+        // dense waves (most lanes in cells of >= 8 agents) take rounds of
+        // kGroupL candidates and skip the self test outside the agent's own
+        // cell; sparse waves, dominated by per-cell overhead, take the short
+        // rounds of kGroupR with one code path
+        bool dense = false;
+        if (LDSL) {
+            uint32_t occ = 0;
+            if (alive) {
+                const uint32_t key = a.hash[L.i].x;
+                occ = a.cell_end[key] - a.cell_start[key];
+            }
+            dense = __popcll(__ballot(alive && occ >= 8u)) >= 32;
+        }
+        if (alive) {
+            if (dense)
+                scan_vector<LDSL, kGroupL, BRT_PROX_SELF_SPLIT != 0>(a, sg, L, gp);
+            else
+                scan_vector<LDSL, kGroupR, false>(a, sg, L, gp);
+        }
     }
 
     if (LDSL || alive) finish_list<LDSL>(L, a.W);
