@@ -52,6 +52,11 @@
 // 1: in dense waves with LDS lists, cells that cannot hold the agent itself run
 // rounds without the self test (a second copy of the round code, one ballot
 // per cell; elsewhere that measured slower); 0: every round tests for the agent.
+// Neighbours gathered together (positions and headings in flight) by the fused
+// steering walk over a finished list.
+#ifndef BRT_PROX_STEER_GROUP
+#define BRT_PROX_STEER_GROUP 4
+#endif
 #ifndef BRT_PROX_SELF_SPLIT
 #define BRT_PROX_SELF_SPLIT 1
 #endif
@@ -483,16 +488,17 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
             if (wts.s != 0.0f || wts.a != 0.0f || wts.c != 0.0f) {
                 const __amdgpu_buffer_rsrc_t rpos = float4_rsrc(a.spos, a.n);
                 const __amdgpu_buffer_rsrc_t rfwd = float4_rsrc(a.sfwd, a.n);
-                for (uint32_t t = 0; t < L.cnt; t += 4) {
-                    float4 q[4], f[4];
+                constexpr int SG = BRT_PROX_STEER_GROUP;
+                for (uint32_t t = 0; t < L.cnt; t += SG) {
+                    float4 q[SG], f[SG];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < SG; ++u) {
                         const uint32_t j = (t + u < L.cnt) ? L.row[1 + t + u] : L.i;
                         q[u] = bload(rpos, j * 16u);
                         f[u] = bload(rfwd, j * 16u);
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < SG; ++u) {
                         if (t + u < L.cnt) {
                             const V3 offset = xyz(q[u]) - myPos;
                             acc.add(a.sp, wts, myFwd, offset, dot(offset, offset), xyz(f[u]));
