@@ -52,13 +52,30 @@
 // 1: in dense waves with LDS lists, cells that cannot hold the agent itself run
 // rounds without the self test (a second copy of the round code, one ballot
 // per cell; elsewhere that measured slower); 0: every round tests for the agent.
+#ifndef BRT_PROX_SELF_SPLIT
+#define BRT_PROX_SELF_SPLIT 1
+#endif
 // Neighbours gathered together (positions and headings in flight) by the fused
 // steering walk over a finished list.
 #ifndef BRT_PROX_STEER_GROUP
 #define BRT_PROX_STEER_GROUP 4
 #endif
-#ifndef BRT_PROX_SELF_SPLIT
-#define BRT_PROX_SELF_SPLIT 1
+// 1: the loops over the 27 neighbour cells stay loops (one copy of the round
+// code per variant; a slab's cell ranges are read from registers by a
+// wave-uniform index); 0: they are unrolled, one copy of the round code per cell.
+#ifndef BRT_PROX_ROLLED
+#define BRT_PROX_ROLLED 1
+#endif
+// 1: LDS lists are built through a per-round log of accept bits (see
+// log_round) when the launch has fewer than 2^24 agents; 0: every accepted
+// candidate is appended as it is tested.
+#ifndef BRT_PROX_LOG
+#define BRT_PROX_LOG 1
+#endif
+#if BRT_PROX_ROLLED
+#define BRT_CELL_LOOP _Pragma("nounroll")
+#else
+#define BRT_CELL_LOOP _Pragma("unroll")
 #endif
 
 namespace brt {
@@ -90,6 +107,7 @@ struct ProxArgs {
     float slack[3];     // per-axis widening of cell boxes for rounding (single grid)
     uint32_t uniform_max_cells;
     uint32_t xcd_remap;
+    uint32_t log_rounds;  // LDS lists through the round log (n < 2^24)
     const float4* spos;
     const float4* sfwd;
     const float4* ssa;
@@ -180,8 +198,13 @@ __device__ __forceinline__ void cell_gaps(const Seg& sg, Lane& L, I3 gp) {
     }
 }
 
+// The lo or hi term of axis ax for offset d (0 for d = 0), chosen by bit masks
+// rather than a select of the two fields, which the compiler may turn into a
+// select of their addresses and so keep the lane's state in scratch memory
+// when d is not a compile-time constant.
 __device__ __forceinline__ float gap2(const Lane& L, int ax, int d) {
-    return d < 0 ? L.lo2[ax] : (d > 0 ? L.hi2[ax] : 0.0f);
+    const uint32_t mlo = d < 0 ? 0xFFFFFFFFu : 0u, mhi = d > 0 ? 0xFFFFFFFFu : 0u;
+    return __uint_as_float((__float_as_uint(L.lo2[ax]) & mlo) | (__float_as_uint(L.hi2[ax]) & mhi));
 }
 
 // Appends sorted index c to the lane's list when take.  LDS lists: one LDS
@@ -271,9 +294,77 @@ __device__ __forceinline__ void test_round(const ProxArgs& a, Lane& L, uint32_t 
     }
 }
 
+// m = 2m + bit in one VALU add-with-carry whose carry-in is the wave's mask of
+// the compare that produced bit (the ballot folds into that compare).
+__device__ __forceinline__ uint32_t shift_in(uint32_t m, bool bit) {
+    const uint64_t c = __builtin_amdgcn_ballot_w64(bit);
+    uint32_t r;
+    uint64_t carry_out;
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(m), "s"(c));
+    return r;
+}
+
+// Round log (LDS lists, sorted indices below 2^24): the round's accept bits are
+// collected with one add-with-carry per candidate (candidate j+t ends in bit
+// 7-t), the limit and self tests are applied to the round's bits at once, and a
+// round with any bit set appends the word j << 8 | bits to the agent's row.
+// The running count includes every accepted bit; a lane stops before a round
+// once it holds K, so at most K words are appended (each carries at least one
+// bit) and they fit in the row.  expand_log turns the words into the list.
+template <int G, bool SELF>
+__device__ __forceinline__ void log_round(const ProxArgs& a, Lane& L, uint32_t j, uint32_t rem, uint32_t di,
+                                          const float4* q) {
+    static_assert(G >= 1 && G <= 8, "a round's bits must fit in 8 bits");
+    uint32_t m = 0;
+#pragma unroll
+    for (int t = 0; t < G; ++t) {
+        const float rx = q[t].x - L.px, ry = q[t].y - L.py, rz = q[t].z - L.pz;
+        const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
+        const float d2 = (xx + yy) + zz;
+        m = shift_in(m, !(d2 > a.T));
+    }
+    // slots t >= rem (rem >= 1) lie past the cell's limit: bits G-1-t below G-rem
+    const uint32_t r = rem < (uint32_t)G ? rem : (uint32_t)G;
+    m &= ~((1u << ((uint32_t)G - r)) - 1u);
+    if (SELF && di < (uint32_t)G) m &= ~(1u << ((uint32_t)G - 1u - di));
+    m <<= 8 - G;
+    if (m != 0u) {
+        *L.wp = (j << 8) | m;
+        ++L.wp;
+    }
+    L.cnt += __popc(m);
+}
+
+// Expands the round log into the list, last word first and in place: word e
+// sits at row[1+e] and its entries go to row[1+p] for p >= the bits of words
+// 0..e-1 >= e, so no unread word is overwritten.  Entries past K are dropped
+// and the count is clamped to K.
+__device__ __forceinline__ void expand_log(Lane& L, uint32_t K) {
+    const uint32_t nlog = (uint32_t)(L.wp - (L.row + 1u));
+    uint32_t pos = L.cnt;
+    for (uint32_t e = nlog; e-- > 0u;) {
+        const uint32_t w = L.row[1u + e];
+        uint32_t m = w & 0xFFu;
+        const uint32_t j7 = (w >> 8) + 7u;
+        pos -= __popc(m);
+        uint32_t p = pos;
+        while (m != 0u) {
+            const uint32_t b = 31u - __clz(m);  // highest bit: the earliest candidate
+            if (p < K) L.row[1u + p] = j7 - b;
+            m ^= 1u << b;
+            ++p;
+        }
+    }
+    L.cnt = L.cnt < K ? L.cnt : K;
+}
+
 template <bool LDSL, bool SELF, int G>
 __device__ __forceinline__ void test_round_any(const ProxArgs& a, Lane& L, uint32_t j, uint32_t rem, uint32_t di,
                                                const float4* q) {
+    if (LDSL && a.log_rounds) {
+        log_round<G, SELF>(a, L, j, rem, di, q);
+        return;
+    }
     if (__ballot(L.cnt + (uint32_t)G > a.K) == 0)
         test_round<LDSL, 0, SELF, G>(a, L, j, rem, di, q);
     else if (LDSL && a.Ws >= a.K + (uint32_t)G)
@@ -303,10 +394,11 @@ __device__ __forceinline__ void scan_uniform(const ProxArgs& a, const Seg& sg, L
     cuint_ptr cstart = (cuint_ptr)a.cell_start;
     cuint_ptr cend = (cuint_ptr)a.cell_end;
     const uint32_t maxBodies = g.max_bodies_per_cell;
-#pragma unroll
+    BRT_CELL_LOOP
     for (int dx = -1; dx <= 1; ++dx) {
+        BRT_CELL_LOOP
         for (int dy = -1; dy <= 1; ++dy) {
-#pragma unroll
+            BRT_CELL_LOOP
             for (int dz = -1; dz <= 1; ++dz) {
                 const int cx = ux + dx, cy = uy + dy, cz = uz + dz;
                 if (!inGrid(cx, cy, cz, g)) continue;
@@ -348,7 +440,7 @@ __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, La
     const brt_grid_params& g = sg.g;
     const uint32_t maxBodies = g.max_bodies_per_cell;
     const __amdgpu_buffer_rsrc_t rpos = float4_rsrc(a.spos, a.n);
-#pragma unroll
+    BRT_CELL_LOOP
     for (int dx = -1; dx <= 1; ++dx) {
         const int cx = gp.x + dx;
         uint32_t st[9], en[9];
@@ -361,7 +453,7 @@ __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, La
             en[k] = a.cell_end[key];
             st[k] = ok ? st[k] : BRT_EMPTY_CELL;
         }
-#pragma unroll
+        BRT_CELL_LOOP
         for (int k = 0; k < 9; ++k) {
             const int dy = k / 3 - 1, dz = k % 3 - 1;
             const uint32_t s = st[k];
@@ -403,6 +495,7 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
     L.cnt = 0;
     const bool alive = wave_live && L.i < sg.end;
     L.row = LDSL ? s_list + tid * a.Ws : a.lists + (size_t)(alive ? L.i : 0u) * a.W;
+    L.wp = L.row + 1u;
     L.h0 = L.h1 = L.h2 = L.b0 = L.b1 = L.b2 = BRT_LIST_PAD;
     float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
     if (alive) P = a.spos[L.i];
@@ -457,6 +550,7 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
         }
     }
 
+    if (LDSL && a.log_rounds) expand_log(L, a.K);
     if (LDSL || alive) finish_list<LDSL>(L, a.W);
     if (LDSL) __syncthreads();
     // LDS lists: each wave writes its own rows, 16 bytes per lane per store
@@ -606,6 +700,7 @@ ProxArgs make_args(const brt_grid_params& g, const brt_prox_params& p, float thr
     cell_slack(g, a.slack);
     a.uniform_max_cells = uniform_max_cells;
     a.xcd_remap = xcd_remap;
+    a.log_rounds = n < (1u << 24) ? BRT_PROX_LOG : 0u;
     a.spos = spos;
     a.hash = hash;
     a.cell_start = cell_start;
