@@ -72,6 +72,10 @@
 #ifndef BRT_PROX_LOG
 #define BRT_PROX_LOG 1
 #endif
+// 1: dense waves with the round log scan in flat mode (see scan_flat).
+#ifndef BRT_PROX_FLAT
+#define BRT_PROX_FLAT 1
+#endif
 #if BRT_PROX_ROLLED
 #define BRT_CELL_LOOP _Pragma("nounroll")
 #else
@@ -480,6 +484,80 @@ __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, La
     }
 }
 
+// Flat mode (LDS round log): each lane walks its own canonical stream with its
+// own cell index, so a lane that finishes a cell goes on to its next cell while
+// the others are still in theirs.  The wave then runs about as many rounds as
+// its longest stream needs, instead of the sum over the 27 cells of the longest
+// stream in each.  The cells to visit (in the grid and not skipped) are a
+// 27-bit mask computed up front; the range of a lane's next cell is loaded
+// while it scans the current one.
+// Takes the next cell of the mask todo: its ordinal nc (27 when none is left)
+// and its cell range, whose loads are left in flight.
+__device__ __forceinline__ void flat_fetch(const ProxArgs& a, const Seg& sg, I3 gp, uint32_t& todo, uint32_t& nc,
+                                           uint32_t& ns, uint32_t& ne) {
+    const bool any = todo != 0u;
+    nc = any ? (uint32_t)__builtin_ctz(todo) : 27u;
+    todo &= todo - 1u;
+    const uint32_t q3 = (nc * 11u) >> 5;  // nc / 3 for nc < 27
+    const uint32_t q9 = (nc * 57u) >> 9;  // nc / 9 for nc < 27
+    const int dx = (int)q9 - 1, dy = (int)(q3 - 3u * q9) - 1, dz = (int)(nc - 3u * q3) - 1;
+    // with no cell left the loads read cell 0 of the grid and are not used
+    const uint32_t key = sg.cell_base + (any ? cellKey(gp.x + dx, gp.y + dy, gp.z + dz, sg.g) : 0u);
+    ns = a.cell_start[key];
+    ne = a.cell_end[key];
+}
+
+template <int G>
+__device__ __forceinline__ void scan_flat(const ProxArgs& a, const Seg& sg, Lane& L, I3 gp) {
+    const brt_grid_params& g = sg.g;
+    const uint32_t maxBodies = g.max_bodies_per_cell;
+    const __amdgpu_buffer_rsrc_t rpos = float4_rsrc(a.spos, a.n);
+    uint32_t todo = 0;
+#pragma unroll
+    for (int c = 0; c < 27; ++c) {
+        const int dx = c / 9 - 1, dy = (c / 3) % 3 - 1, dz = c % 3 - 1;
+        const int cx = gp.x + dx, cy = gp.y + dy, cz = gp.z + dz;
+        const float bd = (gap2(L, 0, dx) + gap2(L, 1, dy)) + gap2(L, 2, dz);
+        const bool visit = inGrid(cx, cy, cz, g) && !(bd > a.Tskip && interior(cx, cy, cz, g));
+        todo |= visit ? (1u << c) : 0u;
+    }
+    // the fetched (next) cell: ordinal nc (27: none left) and its range
+    uint32_t nc = 27u, ns = BRT_EMPTY_CELL, ne = 0u;
+    flat_fetch(a, sg, gp, todo, nc, ns, ne);
+    uint32_t j = 0u, lim = 0u;
+    bool live = true;
+    while (true) {
+        if (live && j >= lim) {
+            if (nc == 27u) {
+                live = false;
+            } else {
+                if (ns == BRT_EMPTY_CELL) {
+                    j = 0u;
+                    lim = 0u;
+                } else {
+                    lim = (ne - ns) > maxBodies ? ns + maxBodies : ne;
+                    lim = lim < a.n ? lim : a.n;
+                    j = ns;
+                }
+                // the old range is consumed before the next one is loaded into
+                // the same registers (no copy of an in-flight load)
+                __builtin_amdgcn_sched_barrier(0);
+                flat_fetch(a, sg, gp, todo, nc, ns, ne);
+            }
+        }
+        if (__ballot(live) == 0) break;
+        if (live && j < lim) {
+            // candidates past lim are read (zeros past n) and not taken
+            float4 q[G];
+#pragma unroll
+            for (int t = 0; t < G; ++t) q[t] = bload(rpos, j * 16u + 16u * (uint32_t)t);
+            log_round<G, true>(a, L, j, lim - j, L.i - j, q);
+            j += (uint32_t)G;
+            if (L.cnt >= a.K) live = false;
+        }
+    }
+}
+
 template <bool FUSE, bool BATCH, bool LDSL>
 __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
     extern __shared__ uint32_t s_list[];  // blockDim.x * W with LDS lists, else unused
@@ -542,7 +620,9 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
             }
             dense = __popcll(__ballot(alive && occ >= 8u)) >= 32;
         }
-        if (alive) {
+        if (LDSL && BRT_PROX_FLAT && dense && a.log_rounds) {
+            if (alive) scan_flat<kGroupL>(a, sg, L, gp);
+        } else if (alive) {
             if (dense)
                 scan_vector<LDSL, kGroupL, BRT_PROX_SELF_SPLIT != 0>(a, sg, L, gp);
             else
