@@ -20,13 +20,15 @@
 //
 // MI355X design.  One lane per sorted agent, 256 lanes per workgroup; lists are
 // staged in LDS and written out with 16-byte coalesced stores.  Per wave64:
-//   * uniform mode (few distinct cells in the wave, the dense case): lanes are
-//     grouped by cell; a group walks the shared canonical candidate stream with
-//     wave-uniform indices, so candidate positions come through the scalar
-//     cache (s_load) instead of 64 per-lane vector loads, four at a time;
-//   * vector mode (many cells, the sparse case): each lane walks its own stream,
-//     four candidate loads in flight per round.
-// Both modes skip a cell whose box lies farther than r from the agent (with
+//   * uniform mode (all lanes in few cells): lanes are grouped by cell; a group
+//     walks the shared canonical candidate stream with wave-uniform indices, so
+//     candidate positions come through the scalar cache (s_load);
+//   * flat mode (LDS lists): each lane walks its own stream of cells with its
+//     own cell index, eight (dense waves) or four (sparse waves) candidate
+//     loads in flight per round, accept bits logged per round;
+//   * vector mode (register-chunk lists): each lane walks its own stream, the
+//     wave in step cell by cell.
+// Every mode skips a cell whose box lies farther than r from the agent (with
 // rounding margins so no candidate that passes the exact test is ever skipped),
 // only for interior cells: boundary cells can hold clamped out-of-grid or NaN
 // agents whose positions are not inside the cell's box.
@@ -75,6 +77,10 @@
 // 1: dense waves with the round log scan in flat mode (see scan_flat).
 #ifndef BRT_PROX_FLAT
 #define BRT_PROX_FLAT 1
+#endif
+// 1: sparse waves with the round log scan in flat mode too.
+#ifndef BRT_PROX_FLAT_SPARSE
+#define BRT_PROX_FLAT_SPARSE 1
 #endif
 #if BRT_PROX_ROLLED
 #define BRT_CELL_LOOP _Pragma("nounroll")
@@ -622,6 +628,8 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
         }
         if (LDSL && BRT_PROX_FLAT && dense && a.log_rounds) {
             if (alive) scan_flat<kGroupL>(a, sg, L, gp);
+        } else if (LDSL && BRT_PROX_FLAT_SPARSE && a.log_rounds) {
+            if (alive) scan_flat<kGroupR>(a, sg, L, gp);
         } else if (alive) {
             if (dense)
                 scan_vector<LDSL, kGroupL, BRT_PROX_SELF_SPLIT != 0>(a, sg, L, gp);
