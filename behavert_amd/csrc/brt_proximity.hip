@@ -497,6 +497,42 @@ __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, La
 // stream in each.  The cells to visit (in the grid and not skipped) are a
 // 27-bit mask computed up front; the range of a lane's next cell is loaded
 // while it scans the current one.
+// Bit d+1 set when coordinate c+d (d = -1, 0, 1) lies in [lo, hi].
+__device__ __forceinline__ uint32_t offset_bits(int c, int lo, int hi) {
+    return ((c - 1 >= lo && c - 1 <= hi) ? 1u : 0u) | ((c >= lo && c <= hi) ? 2u : 0u) |
+           ((c + 1 >= lo && c + 1 <= hi) ? 4u : 0u);
+}
+
+// 27-bit masks over the cell ordinals c = 9(dx+1) + 3(dy+1) + (dz+1) from the
+// per-axis offset bits: the x bits fill blocks of nine, the y bits groups of
+// three in every block, the z bits one position in every group.
+__device__ __forceinline__ uint32_t cells_of(uint32_t bx, uint32_t by, uint32_t bz) {
+    const uint32_t x = ((bx & 1u) | ((bx & 2u) << 8) | ((bx & 4u) << 16)) * 0x1FFu;
+    const uint32_t y = (((by & 1u) | ((by & 2u) << 2) | ((by & 4u) << 4)) * 7u) * 0x40201u;
+    const uint32_t z = (bz * 0x49u) * 0x40201u;
+    return x & y & z;
+}
+
+// The cells of the 27-neighbourhood a lane visits: in the grid, and not an
+// interior cell whose box lies farther than r (same tests as the cell-by-cell
+// scan, without branches).
+__device__ __forceinline__ uint32_t visit_mask(const Seg& sg, const Lane& L, I3 gp, float Tskip) {
+    const brt_grid_params& g = sg.g;
+    const int gx = (int)g.grid_size[0], gy = (int)g.grid_size[1], gz = (int)g.grid_size[2];
+    const uint32_t in_grid = cells_of(offset_bits(gp.x, 0, gx - 1), offset_bits(gp.y, 0, gy - 1),
+                                      offset_bits(gp.z, 0, gz - 1));
+    const uint32_t inner = cells_of(offset_bits(gp.x, 1, gx - 2), offset_bits(gp.y, 1, gy - 2),
+                                    offset_bits(gp.z, 1, gz - 2));
+    uint32_t far = 0;
+#pragma unroll
+    for (int c = 0; c < 27; ++c) {
+        const int dx = c / 9 - 1, dy = (c / 3) % 3 - 1, dz = c % 3 - 1;
+        const float bd = (gap2(L, 0, dx) + gap2(L, 1, dy)) + gap2(L, 2, dz);
+        far |= bd > Tskip ? (1u << c) : 0u;
+    }
+    return in_grid & ~(far & inner);
+}
+
 // Takes the next cell of the mask todo: its ordinal nc (27 when none is left)
 // and its cell range, whose loads are left in flight.
 __device__ __forceinline__ void flat_fetch(const ProxArgs& a, const Seg& sg, I3 gp, uint32_t& todo, uint32_t& nc,
@@ -518,15 +554,7 @@ __device__ __forceinline__ void scan_flat(const ProxArgs& a, const Seg& sg, Lane
     const brt_grid_params& g = sg.g;
     const uint32_t maxBodies = g.max_bodies_per_cell;
     const __amdgpu_buffer_rsrc_t rpos = float4_rsrc(a.spos, a.n);
-    uint32_t todo = 0;
-#pragma unroll
-    for (int c = 0; c < 27; ++c) {
-        const int dx = c / 9 - 1, dy = (c / 3) % 3 - 1, dz = c % 3 - 1;
-        const int cx = gp.x + dx, cy = gp.y + dy, cz = gp.z + dz;
-        const float bd = (gap2(L, 0, dx) + gap2(L, 1, dy)) + gap2(L, 2, dz);
-        const bool visit = inGrid(cx, cy, cz, g) && !(bd > a.Tskip && interior(cx, cy, cz, g));
-        todo |= visit ? (1u << c) : 0u;
-    }
+    uint32_t todo = visit_mask(sg, L, gp, a.Tskip);
     // the fetched (next) cell: ordinal nc (27: none left) and its range
     uint32_t nc = 27u, ns = BRT_EMPTY_CELL, ne = 0u;
     flat_fetch(a, sg, gp, todo, nc, ns, ne);
