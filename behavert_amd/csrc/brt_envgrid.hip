@@ -184,7 +184,10 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
         const bool valid = (wbase + k * 64 + lane) < n;
         const uint32_t d = (it[k].x >> shift) & mask;
         uint64_t m = __ballot(valid);
-        for (uint32_t b = 0; b < nbits; ++b) {
+        // eight bits whatever nbits is: bits above nbits are zero in every lane
+        // and leave the match mask unchanged
+#pragma unroll
+        for (uint32_t b = 0; b < 8u; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bal = __ballot(bit);
             m &= bit ? bal : ~bal;
@@ -259,11 +262,24 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
     __syncthreads();
     const uint32_t tile_begin = tile * TILE;
     const uint32_t tile_n = (n - tile_begin) < (uint32_t)TILE ? (n - tile_begin) : (uint32_t)TILE;
-    for (uint32_t j = tid; j < tile_n; j += kSortThreads) {
-        const uint2 v = s_items[j];
-        const uint32_t d = (v.x >> shift) & mask;
-        const uint32_t g = s_base[d] + (j - s_digit_start[d]);
-        if (g < n) out[g] = v;
+    if (tile_n == (uint32_t)TILE) {
+        // full tile: the ITEMS slots of a thread are independent, so their LDS
+        // reads overlap
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t j = (uint32_t)(k * kSortThreads + tid);
+            const uint2 v = s_items[j];
+            const uint32_t d = (v.x >> shift) & mask;
+            const uint32_t g = s_base[d] + (j - s_digit_start[d]);
+            if (g < n) out[g] = v;
+        }
+    } else {
+        for (uint32_t j = tid; j < tile_n; j += kSortThreads) {
+            const uint2 v = s_items[j];
+            const uint32_t d = (v.x >> shift) & mask;
+            const uint32_t g = s_base[d] + (j - s_digit_start[d]);
+            if (g < n) out[g] = v;
+        }
     }
 }
 
@@ -281,6 +297,14 @@ __global__ void __launch_bounds__(256) k_reorder_ranges(const uint2* __restrict_
         if (i == n - 1 || hash[i + 1].x != h.x) cell_end[h.x] = i + 1;
     }
 }
+
+// Workgroups of the hash + histogram kernel: each adds its per-workgroup
+// histograms of every pass into the global ones with one atomic per nonzero bin.
+// 256 measured fastest on gfx950 for 1M agents (12.5 us against 24.6 us with
+// 1024 workgroups: the global atomics of the flush dominate).
+#ifndef BRT_HIST_BLOCKS
+#define BRT_HIST_BLOCKS 256
+#endif
 
 inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t cap) {
     uint32_t g = (n + block - 1) / block;
@@ -320,14 +344,14 @@ hipError_t launch_prepare(uint32_t* meta, size_t meta_words, uint32_t* cell_star
 
 hipError_t launch_hash_hist(const brt_grid_params& g, const float4* pos, uint32_t n, uint2* out, uint32_t* meta,
                             const SortPlan& plan, hipStream_t s) {
-    uint32_t grid = grid_for(n, 256, 1024);
+    uint32_t grid = grid_for(n, 256, BRT_HIST_BLOCKS);
     hipLaunchKernelGGL(k_hash_hist, dim3(grid), dim3(256), 0, s, g, pos, n, out, meta + kMetaHist, plan);
     return hipGetLastError();
 }
 
 hipError_t launch_batch_hash_hist(const BatchView& b, const float4* pos, uint2* out, uint32_t* meta,
                                   const SortPlan& plan, hipStream_t s) {
-    uint32_t grid = grid_for(b.waves, 4, 1024);
+    uint32_t grid = grid_for(b.waves, 4, BRT_HIST_BLOCKS);
     hipLaunchKernelGGL(k_batch_hash_hist, dim3(grid), dim3(256), 0, s, b, pos, out, meta + kMetaHist, plan);
     return hipGetLastError();
 }
@@ -339,7 +363,7 @@ hipError_t launch_set_word(uint32_t* p, uint32_t v, hipStream_t s) {
 
 hipError_t launch_hist(const uint2* in, uint2* copy_out, uint32_t n, uint32_t* meta, const SortPlan& plan,
                        hipStream_t s) {
-    uint32_t grid = grid_for(n, 256, 1024);
+    uint32_t grid = grid_for(n, 256, BRT_HIST_BLOCKS);
     hipLaunchKernelGGL(k_hist, dim3(grid), dim3(256), 0, s, in, copy_out, n, meta + kMetaHist, plan);
     return hipGetLastError();
 }
