@@ -9,9 +9,10 @@
 //   s1 = |v|, p1 = p + v dt, f1 = normalize(v).
 // Given dL/d(p1, f1, s1, a1) by agent ID this kernel recomputes the forward
 // values of agent i (same arithmetic as the forward kernels), back-propagates
-// through the integrator and the behaviour sums, and scatters dL/d(p, f, s, a)
-// to agent i and to its neighbours j (p_j, f_j appear in i's sums) with
-// hardware float atomics.  The radius/angle predicates, the list membership and
+// through the integrator and the behaviour sums, writes agent i's own terms of
+// dL/d(p, f, s, a) and adds its neighbours' terms (p_j, f_j appear in i's sums)
+// into a scratch accumulator with hardware float atomics; a second kernel adds
+// the accumulator to the outputs (only the neighbour terms need atomics).  The radius/angle predicates, the list membership and
 // the world-box test are piecewise constant and carry no gradient.  dL/dw for
 // the three behaviour weights is reduced per workgroup and then in a fixed order.
 // The reference has no adjoint; the forward it differentiates is the one
@@ -60,6 +61,8 @@ struct AdjArgs {
     float4* gpos_in;
     float4* gfwd_in;
     float4* gsa_in;
+    float4* nbr_pos;         // neighbour terms of dL/dp_j by sorted index j, accumulated with atomics
+    float4* nbr_fwd;         // neighbour terms of dL/df_j by sorted index j
     float* partials;         // per workgroup: 3 floats
     uint32_t* done;          // workgroups finished (reset to 0 by the last one)
     float* weight_grad;
@@ -280,7 +283,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
         }
         if (behaviours)
             for (uint32_t t = 0; t < count; t += 4) {
-                uint32_t jj[4], jid4[4];
+                uint32_t jj[4];
                 float4 q[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -288,14 +291,11 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
                     jj[u] = j;
                     const uint32_t js = j < a.n ? j : i;
                     q[u] = a.spos[js];
-                    jid4[u] = a.hash[js].y;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const uint32_t j = jj[u];
                     if (j >= a.n) continue;
-                    const uint32_t jid = jid4[u];
-                    if (jid >= a.n) continue;
                     const V3 off = xyz(q[u]) - p;
                     const float d2 = dot(off, off);
                     const float dist = sqrtf(d2);
@@ -316,18 +316,19 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
                         any = true;
                     }
                     if (any) {
-                        atomic_add3(&a.gpos_in[jid], offbar);
+                        atomic_add3(&a.nbr_pos[j], offbar);
                         pbar = pbar - offbar;
                     }
                     if (acc.nAli > 0 && sp.alignment[0] != 0.0f &&
                         inBoid(dist, cosA, minR, sp.alignment[1], sp.alignment[2]))
-                        atomic_add3(&a.gfwd_in[jid], Abar);
+                        atomic_add3(&a.nbr_fwd[j], Abar);
                 }
             }
-        atomic_add3(&a.gpos_in[id], pbar);
-        atomic_add3(&a.gfwd_in[id], fbar);
-        unsafeAtomicAdd(reinterpret_cast<float*>(&a.gfwd_in[id]) + 3, sbar);
-        atomic_add3(&a.gsa_in[id], abar);
+        // the agent's own terms are written once (its ID appears once in the
+        // sorted hash); the neighbours' terms are added by k_adjoint_combine
+        a.gpos_in[id] = make_float4(pbar.x, pbar.y, pbar.z, 0.0f);
+        a.gfwd_in[id] = make_float4(fbar.x, fbar.y, fbar.z, sbar);
+        a.gsa_in[id] = make_float4(abar.x, abar.y, abar.z, 0.0f);
     }
     // workgroup reduction of dL/dw in a fixed order
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -359,14 +360,39 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
 }
 
 __global__ void __launch_bounds__(256) k_zero3(float4* __restrict__ a, float4* __restrict__ b,
-                                               float4* __restrict__ c, uint32_t n) {
+                                               float4* __restrict__ c, float4* __restrict__ d,
+                                               uint32_t n) {
     const uint32_t stride = gridDim.x * blockDim.x;
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         a[i] = z;
         b[i] = z;
         c[i] = z;
+        d[i] = z;
+        d[n + i] = z;
     }
+}
+
+// dL/d(p, f) of each agent: its own terms plus the neighbours' terms, which
+// were accumulated by sorted index (neighbours are close in sorted order, so
+// the atomics of a wave share cache lines)
+__global__ void __launch_bounds__(256) k_adjoint_combine(float4* __restrict__ gpos_in, float4* __restrict__ gfwd_in,
+                                                         const float4* __restrict__ nbr, const uint2* __restrict__ hash,
+                                                         uint32_t n) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t id = hash[j].y;
+    if (id >= n) return;
+    const float4 np = nbr[j], nf = nbr[n + j];
+    float4 p = gpos_in[id], f = gfwd_in[id];
+    p.x += np.x;
+    p.y += np.y;
+    p.z += np.z;
+    f.x += nf.x;
+    f.y += nf.y;
+    f.z += nf.z;
+    gpos_in[id] = p;
+    gfwd_in[id] = f;
 }
 
 }  // namespace
@@ -375,11 +401,11 @@ hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_
                           const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
                           const uint32_t* lists, const float4* gpos_out, const float4* gfwd_out,
                           const float4* gsa_out, float4* gpos_in, float4* gfwd_in, float4* gsa_in,
-                          float* partials, uint32_t* done, float* weight_grad, uint32_t xcd_remap,
+                          float4* nbr, float* partials, uint32_t* done, float* weight_grad, uint32_t xcd_remap,
                           hipStream_t s) {
     const uint32_t blocks = (n + 255) / 256;
     uint32_t zgrid = blocks < 2048 ? blocks : 2048;
-    hipLaunchKernelGGL(k_zero3, dim3(zgrid), dim3(256), 0, s, gpos_in, gfwd_in, gsa_in, n);
+    hipLaunchKernelGGL(k_zero3, dim3(zgrid), dim3(256), 0, s, gpos_in, gfwd_in, gsa_in, nbr, n);
     AdjArgs a{};
     a.g = g;
     a.sp = sp;
@@ -397,11 +423,14 @@ hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_
     a.gpos_in = gpos_in;
     a.gfwd_in = gfwd_in;
     a.gsa_in = gsa_in;
+    a.nbr_pos = nbr;
+    a.nbr_fwd = nbr + n;
     a.partials = partials;
     a.done = done;
     a.weight_grad = weight_grad;
     a.xcd_remap = xcd_remap;
     hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_adjoint_combine, dim3(blocks), dim3(256), 0, s, gpos_in, gfwd_in, nbr, hash, n);
     return hipGetLastError();
 }
 

@@ -470,6 +470,8 @@ int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid, const br
         ctx->partials_bytes = need;
     }
     uint32_t* done = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ctx->partials) + ctx->partials_bytes - 256);
+    // neighbour-term accumulators (2 x n float4) in the context's scratch
+    if ((rc = ensure_tmp(ctx, (size_t)n * 32u))) return rc;
     BRT_HIP(brt::launch_adjoint(*grid, prox->max_neighbors, *steer, dt, n,
                                 reinterpret_cast<const float4*>(sorted_pos),
                                 reinterpret_cast<const float4*>(sorted_forward),
@@ -479,7 +481,8 @@ int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid, const br
                                 reinterpret_cast<const float4*>(adj_forward_out),
                                 reinterpret_cast<const float4*>(adj_smooth_accel_out),
                                 reinterpret_cast<float4*>(adj_pos_in), reinterpret_cast<float4*>(adj_forward_in),
-                                reinterpret_cast<float4*>(adj_smooth_accel_in), ctx->partials, done,
+                                reinterpret_cast<float4*>(adj_smooth_accel_in),
+                                reinterpret_cast<float4*>(ctx->tmp), ctx->partials, done,
                                 weight_grad, ctx->xcd_remap, (hipStream_t)stream),
             "k_adjoint");
     return BRT_OK;

@@ -110,7 +110,7 @@ hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_
                           const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
                           const uint32_t* lists, const float4* gpos_out, const float4* gfwd_out,
                           const float4* gsa_out, float4* gpos_in, float4* gfwd_in, float4* gsa_in,
-                          float* partials, uint32_t* done, float* weight_grad, uint32_t xcd_remap,
+                          float4* nbr, float* partials, uint32_t* done, float* weight_grad, uint32_t xcd_remap,
                           hipStream_t s);
 
 inline uint32_t list_stride(uint32_t K) { return 4u * (K / 4u + 1u); }
