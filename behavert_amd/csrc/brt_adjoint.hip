@@ -346,11 +346,13 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
         a.partials[(size_t)blockIdx.x * 3 + threadIdx.x] = v;
     }
     // the last workgroup to finish sums every workgroup's partials in a fixed
-    // order (no wait: it only runs once all others have published)
+    // order (no wait: it only runs once all others have published).  The
+    // partials are written by wave 0, so only wave 0 fences before the count.
     __shared__ bool s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(a.done, 1u) == gridDim.x - 1u;
+    if (threadIdx.x < 64u) {
+        __threadfence();
+        if (threadIdx.x == 0) s_last = atomicAdd(a.done, 1u) == gridDim.x - 1u;
+    }
     __syncthreads();
     if (s_last) {
         __threadfence();
