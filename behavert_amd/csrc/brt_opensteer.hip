@@ -17,6 +17,7 @@
 // BehaviorClustering_computeNeighNaiveSimilarities (BehaviorClustering_kernel.cu:306-353).
 #include "brt_device.hpp"
 #include "brt_internal.hpp"
+#include "brt_lists.hpp"
 #include "brt_steer_common.hpp"
 
 namespace brt {
@@ -105,27 +106,30 @@ __global__ void __launch_bounds__(256) k_features(uint32_t W, float rate, uint32
     if (i >= n) return;
     const uint32_t id = hash[i].y;
     if (id >= n) return;
-    const uint4* L = reinterpret_cast<const uint4*>(lists + (size_t)i * W);
-    uint4 word = L[0];
-    const uint32_t count = word.x < W - 1u ? word.x : W - 1u;
     const float4 f = fin[id];
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    // entries eight at a time: their IDs, then their features, in flight
+    // together; summed in list order
+    const uint32_t count =
+        for_list_octets(lists, W, W - 1u, i, n, [&](uint32_t, const uint32_t(&jj)[8], uint32_t mask) {
+            uint32_t jid[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) jid[k] = hash[jj[k]].y;
+            float4 q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] = fin[jid[k] < n ? jid[k] : id];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (((mask >> k) & 1u) == 0u || jid[k] >= n) continue;
+                s.x = s.x + q[k].x;
+                s.y = s.y + q[k].y;
+                s.z = s.z + q[k].z;
+                s.w = s.w + q[k].w;
+            }
+        });
     if (count == 0) {
         fout[id] = f;
         return;
-    }
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (uint32_t t = 0; t < count; ++t) {
-        const uint32_t e = t + 1;
-        if ((e & 3u) == 0u) word = L[e >> 2];
-        const uint32_t sub = e & 3u;
-        const uint32_t j = sub == 0 ? word.x : (sub == 1 ? word.y : (sub == 2 ? word.z : word.w));
-        const uint32_t jid = j < n ? hash[j].y : n;
-        if (jid >= n) continue;
-        const float4 q = fin[jid];
-        s.x = s.x + q.x;
-        s.y = s.y + q.y;
-        s.z = s.z + q.z;
-        s.w = s.w + q.w;
     }
     const float inv = 1.0f / (float)count;  // cutil float4 / float
     float4 r;
