@@ -82,6 +82,11 @@
 #ifndef BRT_PROX_FLAT_SPARSE
 #define BRT_PROX_FLAT_SPARSE 1
 #endif
+// Workgroup of the single-grid LDS-list neighbour kernels for launches of 2^18
+// agents or more (see prox_block).
+#ifndef BRT_PROX_BLOCK_LARGE
+#define BRT_PROX_BLOCK_LARGE 64
+#endif
 #if BRT_PROX_ROLLED
 #define BRT_CELL_LOOP _Pragma("nounroll")
 #else
@@ -793,8 +798,13 @@ __global__ void k_neighbors_knn(brt_grid_params g, uint32_t K, uint32_t W, float
 
 // largest workgroup (256, 128 or 64 lanes) whose list staging fits in 40 KB of
 // LDS (four workgroups per CU), 64 lanes at least
-uint32_t prox_block(uint32_t W) {
-    uint32_t B = 256;
+// Workgroup of the single-grid LDS-list neighbour kernels: one wave for large
+// launches, so that a wave that finishes early releases its LDS tile at once
+// instead of waiting for the slowest wave of a larger workgroup (1M-boid blob
+// flock: 64 lanes 5 % faster than 256); 256 lanes for small launches (65k
+// boids: 3 % faster than 64), halved while the tile exceeds 40 KB.
+uint32_t prox_block(uint32_t W, uint32_t n) {
+    uint32_t B = n >= (1u << 18) ? BRT_PROX_BLOCK_LARGE : 256u;
     while (B > 64 && (size_t)B * W * 4u > 40u * 1024u) B /= 2;
     return B;
 }
@@ -842,7 +852,7 @@ hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, 
         return hipGetLastError();
     }
     ProxArgs a = make_args(g, p, threshold, spos, hash, cell_start, cell_end, n, lists, uniform_max_cells, xcd_remap);
-    const uint32_t B = prox_block(a.Ws);
+    const uint32_t B = prox_block(a.Ws, n);
     if (lds_lists)
         hipLaunchKernelGGL((k_prox<false, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.Ws * 4u, s, a);
     else
@@ -863,7 +873,7 @@ hipError_t launch_neighbors_steer(const brt_grid_params& g, const brt_prox_param
     a.dt = dt;
     a.step = step;
     a.o = AgentOut{pos_out, fwd_out, sa_out, nullptr, wander};
-    const uint32_t B = prox_block(a.Ws);
+    const uint32_t B = prox_block(a.Ws, n);
     if (lds_lists)
         hipLaunchKernelGGL((k_prox<true, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.Ws * 4u, s, a);
     else
@@ -886,7 +896,7 @@ hipError_t launch_batch_neighbors_steer(const BatchView& b, uint32_t n, const br
     a.step = 0;
     a.o = AgentOut{pos_out, fwd_out, sa_out, nullptr, wander};
     a.b = b;
-    const uint32_t B = block ? block : prox_block(a.Ws);
+    const uint32_t B = block ? block : prox_block(a.Ws, 0u);
     const uint32_t wpb = B / 64u;
     if (lds_lists)
         hipLaunchKernelGGL((k_prox<true, true, true>), dim3((b.waves + wpb - 1) / wpb), dim3(B), (size_t)B * a.Ws * 4u,

@@ -277,10 +277,15 @@ def test_rollout_gradient_agreement_by_horizon(ctx, orc, record_property):
 
 
 def test_checkpoint_interval_does_not_change_gradients(ctx, orc):
+    """The replayed segments equal the forward, so the gradients agree up to the
+    order of the adjoint's float atomics: a component summed from terms that
+    cancel can differ by a few ulp of its largest term, so the absolute
+    tolerance is 1e-7 of the largest component (a wrong replay moves
+    components by far more)."""
     res = [rollout_case(ctx, orc, 2048, 50.0, 12, every) for every in (1, 5, 12)]
     for _, _, gw, gv, _, _ in res[1:]:
         np.testing.assert_allclose(gw, res[0][2], rtol=1e-5, atol=1e-9)
-        np.testing.assert_allclose(gv, res[0][3], rtol=1e-4, atol=1e-9 * np.abs(res[0][3]).max())
+        np.testing.assert_allclose(gv, res[0][3], rtol=1e-4, atol=1e-7 * np.abs(res[0][3]).max())
 
 
 def test_adjoint_rejects_unsupported_options(ctx):
