@@ -183,15 +183,20 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
     for (int k = 0; k < ITEMS; ++k) {
         const bool valid = (wbase + k * 64 + lane) < n;
         const uint32_t d = (it[k].x >> shift) & mask;
-        uint64_t m = __ballot(valid);
-        // eight bits whatever nbits is: bits above nbits are zero in every lane
-        // and leave the match mask unchanged
+        const uint64_t vm = __ballot(valid);
+        // lanes whose digit equals this lane's: the AND over the eight bits of
+        // (ballot of bit b) XNOR (this lane's bit b replicated), on the two
+        // 32-bit halves (bits above nbits are zero in every lane and leave the
+        // mask unchanged)
+        uint32_t m_lo = (uint32_t)vm, m_hi = (uint32_t)(vm >> 32);
 #pragma unroll
         for (uint32_t b = 0; b < 8u; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
-            m &= bit ? bal : ~bal;
+            const uint32_t rep = (uint32_t)((int32_t)(d << (31u - b)) >> 31);  // bit b: 0 or ~0
+            const uint64_t bal = __ballot(rep != 0u);
+            m_lo &= ~((uint32_t)bal ^ rep);
+            m_hi &= ~((uint32_t)(bal >> 32) ^ rep);
         }
+        const uint64_t m = ((uint64_t)m_hi << 32) | m_lo;
         const uint32_t cnt = __popcll(m);
         const uint32_t below = __popcll(m & lt);
         const int leader = __ffsll((unsigned long long)m) - 1;
