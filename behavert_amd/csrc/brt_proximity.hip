@@ -796,8 +796,6 @@ __global__ void k_neighbors_knn(brt_grid_params g, uint32_t K, uint32_t W, float
     for (uint32_t w = tid; w < words4; w += B) dst[w] = src[w];
 }
 
-// largest workgroup (256, 128 or 64 lanes) whose list staging fits in 40 KB of
-// LDS (four workgroups per CU), 64 lanes at least
 // Workgroup of the single-grid LDS-list neighbour kernels: one wave for large
 // launches, so that a wave that finishes early releases its LDS tile at once
 // instead of waiting for the slowest wave of a larger workgroup (1M-boid blob
