@@ -87,6 +87,15 @@
 #ifndef BRT_PROX_BLOCK_LARGE
 #define BRT_PROX_BLOCK_LARGE 64
 #endif
+// A wave is dense (rounds of BRT_PROX_GROUP candidates) when at least
+// BRT_PROX_DENSE_LANES of its lanes sit in cells of BRT_PROX_DENSE_OCC agents
+// or more.
+#ifndef BRT_PROX_DENSE_OCC
+#define BRT_PROX_DENSE_OCC 8
+#endif
+#ifndef BRT_PROX_DENSE_LANES
+#define BRT_PROX_DENSE_LANES 32
+#endif
 #if BRT_PROX_ROLLED
 #define BRT_CELL_LOOP _Pragma("nounroll")
 #else
@@ -657,7 +666,7 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
                 const uint32_t key = a.hash[L.i].x;
                 occ = a.cell_end[key] - a.cell_start[key];
             }
-            dense = __popcll(__ballot(alive && occ >= 8u)) >= 32;
+            dense = __popcll(__ballot(alive && occ >= (uint32_t)BRT_PROX_DENSE_OCC)) >= BRT_PROX_DENSE_LANES;
         }
         if (LDSL && BRT_PROX_FLAT && dense && a.log_rounds) {
             if (alive) scan_flat<kGroupL>(a, sg, L, gp);
