@@ -784,7 +784,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
                     aligned16(b->lists) && aligned16(b->features) && aligned16(b->features_out) &&
                     aligned16(b->cell_start) && aligned16(b->cell_end),
                 "float4 buffers must be 16-byte aligned");
-    const bool do_features = b->features && b->features_out && feature_rate != 0.0f;
+    bool do_features = b->features && b->features_out && feature_rate != 0.0f;
     BRT_REQUIRE(!do_features || b->features != b->features_out, "features and features_out must differ");
     BRT_REQUIRE(steer->wander_weight == 0.0f || b->wander != nullptr, "wander_weight != 0 needs the wander state");
     if ((rc = use_device(ctx))) return rc;
@@ -840,7 +840,8 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
                                       b->lists, ctx->uniform_max_cells, ctx->xcd_remap, lds, s),
                 "k_neighbors");
         if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
-        // phase 3: fused steering + integration, scattered back to agent-ID order
+        // phase 3: fused steering + integration, scattered back to agent-ID order,
+        // with the feature update in the same pass over the lists
         BRT_HIP(brt::launch_steer(*grid, prox->max_neighbors, *steer, dt, step, n,
                                   reinterpret_cast<const float4*>(b->sorted_pos),
                                   reinterpret_cast<const float4*>(b->sorted_forward),
@@ -848,11 +849,14 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
                                   reinterpret_cast<const uint2*>(b->hash), b->lists,
                                   reinterpret_cast<float2*>(b->wander), reinterpret_cast<float4*>(b->pos),
                                   reinterpret_cast<float4*>(b->forward), reinterpret_cast<float4*>(b->smooth_accel),
-                                  nullptr, ctx->xcd_remap, s),
+                                  nullptr, ctx->xcd_remap, s, feature_rate,
+                                  do_features ? reinterpret_cast<const float4*>(b->features) : nullptr,
+                                  do_features ? reinterpret_cast<float4*>(b->features_out) : nullptr),
                 "k_steer");
         if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
+        do_features = false;
     }
-    // phase 4: BehaviorClustering feature update
+    // phase 4: BehaviorClustering feature update (fused path)
     if (do_features)
         BRT_HIP(brt::launch_features(prox->max_neighbors, feature_rate, n, reinterpret_cast<const uint2*>(b->hash),
                                      b->lists, reinterpret_cast<const float4*>(b->features),

@@ -28,12 +28,62 @@ namespace {
 // gathers issued together), accumulate the behaviour sums, then assemble the
 // force and integrate (APPLY), or only store the behaviour force in sorted
 // order (the steer-force accumulator of the decomposed sequence).
-template <bool APPLY>
+// The feature update of the agent at sorted index i (ID id):
+// f + (mean of the listed neighbours' features - f) * rate, the mean over the
+// list's count; the entries are visited eight at a time with their ID and
+// feature gathers in flight together and summed in list order.
+__device__ __forceinline__ float4 feature_update(uint32_t W, float rate, uint32_t n, const uint2* __restrict__ hash,
+                                                 const uint32_t* __restrict__ lists, const float4* __restrict__ fin,
+                                                 uint32_t i, uint32_t id) {
+    const float4 f = fin[id];
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t count =
+        for_list_octets(lists, W, W - 1u, i, n, [&](uint32_t, const uint32_t(&jj)[8], uint32_t mask) {
+            uint32_t jid[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) jid[k] = hash[jj[k]].y;
+            float4 q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] = fin[jid[k] < n ? jid[k] : id];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (((mask >> k) & 1u) == 0u || jid[k] >= n) continue;
+                s.x = s.x + q[k].x;
+                s.y = s.y + q[k].y;
+                s.z = s.z + q[k].z;
+                s.w = s.w + q[k].w;
+            }
+        });
+    if (count == 0) return f;
+    const float inv = 1.0f / (float)count;  // cutil float4 / float
+    float4 r;
+    r.x = f.x + (s.x * inv - f.x) * rate;
+    r.y = f.y + (s.y * inv - f.y) * rate;
+    r.z = f.z + (s.z * inv - f.z) * rate;
+    r.w = f.w + (s.w * inv - f.w) * rate;
+    return r;
+}
+
+__global__ void __launch_bounds__(256) k_features(uint32_t W, float rate, uint32_t n, const uint2* __restrict__ hash,
+                                                  const uint32_t* __restrict__ lists,
+                                                  const float4* __restrict__ fin, float4* __restrict__ fout) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = hash[i].y;
+    if (id >= n) return;
+    fout[id] = feature_update(W, rate, n, hash, lists, fin, i, id);
+}
+
+// FEAT: the same pass also writes the agent's feature update (see
+// feature_update), walking its list a second time (from cache).
+template <bool APPLY, bool FEAT = false>
 __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, brt_steer_params sp, float dt,
                                                uint32_t step, uint32_t n, const float4* __restrict__ spos,
                                                const float4* __restrict__ sfwd, const float4* __restrict__ ssa,
                                                const uint2* __restrict__ hash,
-                                               const uint32_t* __restrict__ lists, AgentOut o, uint32_t xcd_remap) {
+                                               const uint32_t* __restrict__ lists, AgentOut o, uint32_t xcd_remap,
+                                               float feature_rate = 0.0f, const float4* __restrict__ fin = nullptr,
+                                               float4* __restrict__ fout = nullptr) {
     const uint32_t i = xcd_block(xcd_remap != 0) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t id = hash[i].y;
@@ -77,6 +127,7 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
             }
         }
     }
+    if (FEAT) fout[id] = feature_update(W, feature_rate, n, hash, lists, fin, i, id);
     if (APPLY) {
         finish_agent(g, sp, wts, dt, step, i, id, id, myPos, myFwd, Fw.w, xyz(SA), SA.w, acc, o);
     } else {
@@ -99,56 +150,22 @@ __global__ void __launch_bounds__(256) k_apply(brt_grid_params g, brt_steer_para
     apply_force(g, sp, dt, i, id, xyz(force_in[i]), xyz(spos[i]), xyz(Fw), Fw.w, xyz(SA), SA.w, o);
 }
 
-__global__ void __launch_bounds__(256) k_features(uint32_t W, float rate, uint32_t n, const uint2* __restrict__ hash,
-                                                  const uint32_t* __restrict__ lists,
-                                                  const float4* __restrict__ fin, float4* __restrict__ fout) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t id = hash[i].y;
-    if (id >= n) return;
-    const float4 f = fin[id];
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    // entries eight at a time: their IDs, then their features, in flight
-    // together; summed in list order
-    const uint32_t count =
-        for_list_octets(lists, W, W - 1u, i, n, [&](uint32_t, const uint32_t(&jj)[8], uint32_t mask) {
-            uint32_t jid[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) jid[k] = hash[jj[k]].y;
-            float4 q[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) q[k] = fin[jid[k] < n ? jid[k] : id];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (((mask >> k) & 1u) == 0u || jid[k] >= n) continue;
-                s.x = s.x + q[k].x;
-                s.y = s.y + q[k].y;
-                s.z = s.z + q[k].z;
-                s.w = s.w + q[k].w;
-            }
-        });
-    if (count == 0) {
-        fout[id] = f;
-        return;
-    }
-    const float inv = 1.0f / (float)count;  // cutil float4 / float
-    float4 r;
-    r.x = f.x + (s.x * inv - f.x) * rate;
-    r.y = f.y + (s.y * inv - f.y) * rate;
-    r.z = f.z + (s.z * inv - f.z) * rate;
-    r.w = f.w + (s.w * inv - f.w) * rate;
-    fout[id] = r;
-}
 
 }  // namespace
 
 hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t step,
                         uint32_t n, const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
                         const uint32_t* lists, float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
-                        float4* force_out, uint32_t xcd_remap, hipStream_t s) {
+                        float4* force_out, uint32_t xcd_remap, hipStream_t s, float feature_rate,
+                        const float4* features_in, float4* features_out) {
     AgentOut o{pos_out, fwd_out, sa_out, force_out, wander};
-    hipLaunchKernelGGL((k_steer<true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt, step, n,
-                       spos, sfwd, ssa, hash, lists, o, xcd_remap);
+    if (features_in)
+        hipLaunchKernelGGL((k_steer<true, true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt,
+                           step, n, spos, sfwd, ssa, hash, lists, o, xcd_remap, feature_rate, features_in,
+                           features_out);
+    else
+        hipLaunchKernelGGL((k_steer<true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt, step,
+                           n, spos, sfwd, ssa, hash, lists, o, xcd_remap, 0.0f, nullptr, nullptr);
     return hipGetLastError();
 }
 
@@ -158,7 +175,7 @@ hipError_t launch_steer_forces(uint32_t K, const brt_steer_params& sp, float dt,
     AgentOut o{nullptr, nullptr, nullptr, force_out, wander};
     brt_grid_params g{};
     hipLaunchKernelGGL((k_steer<false>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt, step, n,
-                       spos, sfwd, sfwd, hash, lists, o, xcd_remap);
+                       spos, sfwd, sfwd, hash, lists, o, xcd_remap, 0.0f, nullptr, nullptr);
     return hipGetLastError();
 }
 

@@ -49,12 +49,16 @@ def algorithmic_bytes_per_agent(w) -> dict:
     }
     if w.features is not None and w.feature_rate:
         b["features"] = 4 + 4 * W + 32  # R ID + list, R own feature 16, W 16
+        # the two-pass step updates the features in the steering pass: the list
+        # and ID are read once for both
+        b["steer_features"] = b["steer_integrate"] + 32
     # fused neighbour lists + steering: the list is written but never re-read
     b["neighbors_steer"] = 16 + 4 * W + 8 * cpa + 36 + 48
     return b
 
 
 PHASE_KERNELS = {"neighbors": "k_prox<false,false,*>", "steer_integrate": "k_steer",
+                 "steer_features": "k_steer<true,true> (steering + feature update)",
                  "hash_sort": "k_prepare+k_hash_hist+k_onesweep", "reorder_ranges": "k_reorder_ranges",
                  "features": "k_features", "neighbors_steer": "k_prox<true,false,*>"}
 
@@ -83,10 +87,15 @@ def phase_table(ctx, w, abytes, fused):
     enabled (HIP events recorded on the step's stream between the phases)."""
     ms, nsteps = ctx.phase_times()
     out = {}
+    merged = not fused and "steer_features" in abytes
     for k, v in ms.items():
         name = "neighbors_steer" if (fused and k == "neighbors") else k
         if fused and k == "steer_integrate":
             continue
+        if merged and k == "features":
+            continue  # computed in the steering pass
+        if merged and k == "steer_integrate":
+            name = "steer_features"
         avg = v / max(nsteps, 1)
         gbs = (abytes[name] * w.n / 1e9) / (avg / 1e3) if avg > 0 and abytes[name] > 0 else 0.0
         out[name] = {"ms": round(avg, 5), "bytes_per_agent": round(abytes[name], 2), "GB/s": round(gbs, 1),
