@@ -102,6 +102,9 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
 
     SteerAccum acc;
     acc.init();
+    // FEAT: the feature sum of the same entries, in the same order as
+    // feature_update (their IDs load with the positions and headings)
+    float4 fsum = make_float4(0.f, 0.f, 0.f, 0.f);
     if (any) {
         // entries e = 1..count live in words e/4; process words in pairs
         for (uint32_t q = 0; 4u * q <= count; q += 2) {
@@ -109,6 +112,7 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
             const uint4 b = (4u * (q + 1) <= count) ? L[q + 1] : make_uint4(0u, 0u, 0u, 0u);
             uint32_t jj[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
             float4 qp[8], qf[8];
+            uint32_t jid[8];
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
                 const uint32_t e = 4u * q + t;
@@ -116,6 +120,7 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
                 const uint32_t j = ok ? jj[t] : i;
                 qp[t] = spos[j];
                 qf[t] = sfwd[j];
+                if (FEAT) jid[t] = hash[j].y;
             }
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
@@ -125,9 +130,38 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
                     acc.add(sp, wts, myFwd, offset, dot(offset, offset), xyz(qf[t]));
                 }
             }
+            if (FEAT) {
+                float4 fq[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) fq[t] = fin[jid[t] < n ? jid[t] : id];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const uint32_t e = 4u * q + t;
+                    if (!(e >= 1u && e <= count && jj[t] < n) || jid[t] >= n) continue;
+                    fsum.x = fsum.x + fq[t].x;
+                    fsum.y = fsum.y + fq[t].y;
+                    fsum.z = fsum.z + fq[t].z;
+                    fsum.w = fsum.w + fq[t].w;
+                }
+            }
         }
     }
-    if (FEAT) fout[id] = feature_update(W, feature_rate, n, hash, lists, fin, i, id);
+    if (FEAT) {
+        if (!any) {
+            fout[id] = feature_update(W, feature_rate, n, hash, lists, fin, i, id);
+        } else {
+            const float4 f = fin[id];
+            float4 r = f;
+            if (count != 0) {
+                const float inv = 1.0f / (float)count;  // cutil float4 / float
+                r.x = f.x + (fsum.x * inv - f.x) * feature_rate;
+                r.y = f.y + (fsum.y * inv - f.y) * feature_rate;
+                r.z = f.z + (fsum.z * inv - f.z) * feature_rate;
+                r.w = f.w + (fsum.w * inv - f.w) * feature_rate;
+            }
+            fout[id] = r;
+        }
+    }
     if (APPLY) {
         finish_agent(g, sp, wts, dt, step, i, id, id, myPos, myFwd, Fw.w, xyz(SA), SA.w, acc, o);
     } else {
