@@ -201,15 +201,27 @@ def test_ten_step_trajectory_within_1e3_of_fp64(ctx, orc, record_property):
         assert err.max() < 1e-3, k
 
 
-def test_full_stack_small_bit_exact(ctx, orc):
+@pytest.mark.parametrize("shape", [(-1, -1), (1, 1)])
+def test_full_stack_small_bit_exact(ctx, orc, shape):
     """Features + wander + moving-target seek at 65,536 agents (the 4M workload's
-    composition at a size the oracle runs in seconds), 3 steps."""
+    composition at a size the oracle runs in seconds), 3 steps, with the
+    automatic step shape at K 64 (two passes, the feature update inside the
+    steering pass) and with the fused neighbour + steering kernel followed by
+    the feature pass."""
+    from behavert_amd.types import OPT_FUSED_STEP, OPT_LDS_LISTS
     w = W.full_stack_4m(n=1 << 16)
     w.side = 640.0 * (1 / 64) ** (1 / 3)
     st = W.uniform_flock(w.n, w.side, 0)
     w.state = st
     w.grid = W.cube_grid(w.side)
-    fl = run_gpu_steps(ctx, w, 3)
+    old = ctx.get_option(OPT_FUSED_STEP), ctx.get_option(OPT_LDS_LISTS)
+    ctx.set_option(OPT_FUSED_STEP, shape[0])
+    ctx.set_option(OPT_LDS_LISTS, shape[1])
+    try:
+        fl = run_gpu_steps(ctx, w, 3)
+    finally:
+        ctx.set_option(OPT_FUSED_STEP, old[0])
+        ctx.set_option(OPT_LDS_LISTS, old[1])
     hf, _ = run_oracle_steps(orc, w, 3)
     got, ref = fl.state(), hf.state()
     for k in ("pos", "forward", "smooth_accel", "features", "wander"):
