@@ -12,9 +12,10 @@
 // through the integrator and the behaviour sums, writes agent i's own terms of
 // dL/d(p, f, s, a) and adds its neighbours' terms (p_j, f_j appear in i's sums)
 // into a scratch accumulator with hardware float atomics; a second kernel adds
-// the accumulator to the outputs (only the neighbour terms need atomics).  The radius/angle predicates, the list membership and
-// the world-box test are piecewise constant and carry no gradient.  dL/dw for
-// the three behaviour weights is reduced per workgroup and then in a fixed order.
+// the accumulator to the outputs (only the neighbour terms need atomics).  The
+// radius/angle predicates, the list membership and the world-box test are
+// piecewise constant and carry no gradient.  dL/dw for the three behaviour
+// weights is reduced per workgroup and then in a fixed order.
 // The reference has no adjoint; the forward it differentiates is the one
 // restated in brt_steer_common.hpp.
 #include "brt_device.hpp"

@@ -74,8 +74,9 @@ __global__ void __launch_bounds__(256) k_features(uint32_t W, float rate, uint32
     fout[id] = feature_update(W, rate, n, hash, lists, fin, i, id);
 }
 
-// FEAT: the same pass also writes the agent's feature update (see
-// feature_update), walking its list a second time (from cache).
+// FEAT: the same pass also writes the agent's feature update (the result of
+// feature_update), summing the neighbours' features in the same list walk as
+// the steering sums; with all steering weights zero it calls feature_update.
 template <bool APPLY, bool FEAT = false>
 __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, brt_steer_params sp, float dt,
                                                uint32_t step, uint32_t n, const float4* __restrict__ spos,

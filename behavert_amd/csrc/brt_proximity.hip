@@ -808,7 +808,7 @@ __global__ void k_neighbors_knn(brt_grid_params g, uint32_t K, uint32_t W, float
 // Workgroup of the single-grid LDS-list neighbour kernels: one wave for large
 // launches, so that a wave that finishes early releases its LDS tile at once
 // instead of waiting for the slowest wave of a larger workgroup (1M-boid blob
-// flock: 64 lanes 5 % faster than 256); 256 lanes for small launches (65k
+// flock: 64 lanes 4-5 % faster than 256); 256 lanes for small launches (65k
 // boids: 3 % faster than 64), halved while the tile exceeds 40 KB.
 uint32_t prox_block(uint32_t W, uint32_t n) {
     uint32_t B = n >= (1u << 18) ? BRT_PROX_BLOCK_LARGE : 256u;
