@@ -801,6 +801,12 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
         ev = &ctx->events[ctx->events_used++];
         BRT_HIP(hipEventRecord((*ev)[0], s), "hipEventRecord");
     }
+    // the two-pass path gathers the features into sorted order with the state
+    // (into the sort's ping-pong buffer, free once the sort is done), so the
+    // steering pass reads the neighbours' features by sorted index
+    const bool two_pass = !(resolve_fused(ctx, prox) && !prox->use_knn);
+    const bool sorted_features = do_features && two_pass;
+    if (sorted_features && (rc = ensure_tmp(ctx, (size_t)n * 16u))) return rc;
     // phase 0: prepare (scratch + empty-cell fill) + hash + sort
     if ((rc = run_hash_sort(ctx, grid, b->pos, n, b->hash, b->cell_start, b->cell_end, s))) return rc;
     if (ev) BRT_HIP(hipEventRecord((*ev)[1], s), "hipEventRecord");
@@ -813,13 +819,18 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
     f.out[1] = reinterpret_cast<float4*>(b->sorted_forward);
     f.in[2] = reinterpret_cast<const float4*>(b->smooth_accel);
     f.out[2] = reinterpret_cast<float4*>(b->sorted_smooth_accel);
+    if (sorted_features) {
+        f.count = 4;
+        f.in[3] = reinterpret_cast<const float4*>(b->features);
+        f.out[3] = reinterpret_cast<float4*>(ctx->tmp);
+    }
     BRT_HIP(brt::launch_reorder_ranges(reinterpret_cast<const uint2*>(b->hash), n, f, grid->num_cells, b->cell_start,
                                        b->cell_end, s),
             "k_reorder_ranges");
     if (ev) BRT_HIP(hipEventRecord((*ev)[2], s), "hipEventRecord");
     const float T = brt_radius_threshold(prox->search_radius);
     const uint32_t lds = resolve_lds_lists(ctx, prox);
-    if (resolve_fused(ctx, prox) && !prox->use_knn) {
+    if (!two_pass) {
         // phase 2: neighbour lists + steering + integration in one pass
         BRT_HIP(brt::launch_neighbors_steer(*grid, *prox, T, *steer, dt, step,
                                             reinterpret_cast<const float4*>(b->sorted_pos),
@@ -851,7 +862,8 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
                                   reinterpret_cast<float4*>(b->forward), reinterpret_cast<float4*>(b->smooth_accel),
                                   nullptr, ctx->xcd_remap, s, feature_rate,
                                   do_features ? reinterpret_cast<const float4*>(b->features) : nullptr,
-                                  do_features ? reinterpret_cast<float4*>(b->features_out) : nullptr),
+                                  do_features ? reinterpret_cast<float4*>(b->features_out) : nullptr,
+                                  sorted_features ? reinterpret_cast<const float4*>(ctx->tmp) : nullptr),
                 "k_steer");
         if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
         do_features = false;

@@ -66,7 +66,8 @@ hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_pa
                         uint32_t n, const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
                         const uint32_t* lists, float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
                         float4* force_out, uint32_t xcd_remap, hipStream_t s, float feature_rate = 0.0f,
-                        const float4* features_in = nullptr, float4* features_out = nullptr);
+                        const float4* features_in = nullptr, float4* features_out = nullptr,
+                        const float4* sorted_features = nullptr);
 hipError_t launch_features(uint32_t K, float rate, uint32_t n, const uint2* hash, const uint32_t* lists,
                            const float4* fin, float4* fout, hipStream_t s);
 // the decomposed steering sequence: behaviour forces only, then the integrator

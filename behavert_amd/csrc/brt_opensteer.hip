@@ -77,14 +77,18 @@ __global__ void __launch_bounds__(256) k_features(uint32_t W, float rate, uint32
 // FEAT: the same pass also writes the agent's feature update (the result of
 // feature_update), summing the neighbours' features in the same list walk as
 // the steering sums; with all steering weights zero it calls feature_update.
-template <bool APPLY, bool FEAT = false>
+// SFEAT: the neighbours' features are read from sfeat, the features in sorted
+// order (sfeat[j] = fin[hash[j].y], gathered with the state), so the walk reads
+// them next to their positions instead of at random agent IDs.
+template <bool APPLY, bool FEAT = false, bool SFEAT = false>
 __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, brt_steer_params sp, float dt,
                                                uint32_t step, uint32_t n, const float4* __restrict__ spos,
                                                const float4* __restrict__ sfwd, const float4* __restrict__ ssa,
                                                const uint2* __restrict__ hash,
                                                const uint32_t* __restrict__ lists, AgentOut o, uint32_t xcd_remap,
                                                float feature_rate = 0.0f, const float4* __restrict__ fin = nullptr,
-                                               float4* __restrict__ fout = nullptr) {
+                                               float4* __restrict__ fout = nullptr,
+                                               const float4* __restrict__ sfeat = nullptr) {
     const uint32_t i = xcd_block(xcd_remap != 0) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t id = hash[i].y;
@@ -112,7 +116,7 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
             const uint4 a = q == 0 ? w0 : L[q];
             const uint4 b = (4u * (q + 1) <= count) ? L[q + 1] : make_uint4(0u, 0u, 0u, 0u);
             uint32_t jj[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-            float4 qp[8], qf[8];
+            float4 qp[8], qf[8], fq[8];
             uint32_t jid[8];
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
@@ -121,7 +125,8 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
                 const uint32_t j = ok ? jj[t] : i;
                 qp[t] = spos[j];
                 qf[t] = sfwd[j];
-                if (FEAT) jid[t] = hash[j].y;
+                if (FEAT && SFEAT) fq[t] = sfeat[j];
+                if (FEAT && !SFEAT) jid[t] = hash[j].y;
             }
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
@@ -132,13 +137,14 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
                 }
             }
             if (FEAT) {
-                float4 fq[8];
+                if (!SFEAT) {
 #pragma unroll
-                for (int t = 0; t < 8; ++t) fq[t] = fin[jid[t] < n ? jid[t] : id];
+                    for (int t = 0; t < 8; ++t) fq[t] = fin[jid[t] < n ? jid[t] : id];
+                }
 #pragma unroll
                 for (int t = 0; t < 8; ++t) {
                     const uint32_t e = 4u * q + t;
-                    if (!(e >= 1u && e <= count && jj[t] < n) || jid[t] >= n) continue;
+                    if (!(e >= 1u && e <= count && jj[t] < n) || (!SFEAT && jid[t] >= n)) continue;
                     fsum.x = fsum.x + fq[t].x;
                     fsum.y = fsum.y + fq[t].y;
                     fsum.z = fsum.z + fq[t].z;
@@ -151,7 +157,7 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
         if (!any) {
             fout[id] = feature_update(W, feature_rate, n, hash, lists, fin, i, id);
         } else {
-            const float4 f = fin[id];
+            const float4 f = SFEAT ? sfeat[i] : fin[id];
             float4 r = f;
             if (count != 0) {
                 const float inv = 1.0f / (float)count;  // cutil float4 / float
@@ -192,12 +198,16 @@ hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_pa
                         uint32_t n, const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
                         const uint32_t* lists, float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
                         float4* force_out, uint32_t xcd_remap, hipStream_t s, float feature_rate,
-                        const float4* features_in, float4* features_out) {
+                        const float4* features_in, float4* features_out, const float4* sorted_features) {
     AgentOut o{pos_out, fwd_out, sa_out, force_out, wander};
-    if (features_in)
+    if (features_in && sorted_features)
+        hipLaunchKernelGGL((k_steer<true, true, true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp,
+                           dt, step, n, spos, sfwd, ssa, hash, lists, o, xcd_remap, feature_rate, features_in,
+                           features_out, sorted_features);
+    else if (features_in)
         hipLaunchKernelGGL((k_steer<true, true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt,
                            step, n, spos, sfwd, ssa, hash, lists, o, xcd_remap, feature_rate, features_in,
-                           features_out);
+                           features_out, nullptr);
     else
         hipLaunchKernelGGL((k_steer<true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt, step,
                            n, spos, sfwd, ssa, hash, lists, o, xcd_remap, 0.0f, nullptr, nullptr);

@@ -49,8 +49,10 @@ def algorithmic_bytes_per_agent(w) -> dict:
     }
     if w.features is not None and w.feature_rate:
         b["features"] = 4 + 4 * W + 32  # R ID + list, R own feature 16, W 16
-        # the two-pass step updates the features in the steering pass: the list
-        # and ID are read once for both
+        # the two-pass step gathers the features into sorted order with the
+        # state (R 16 by ID, W 16) and updates them in the steering pass (the
+        # list is read once for both; R own feature 16, W 16 by ID)
+        b["reorder_ranges_features"] = b["reorder_ranges"] + 32
         b["steer_features"] = b["steer_integrate"] + 32
     # fused neighbour lists + steering: the list is written but never re-read
     b["neighbors_steer"] = 16 + 4 * W + 8 * cpa + 36 + 48
@@ -96,9 +98,10 @@ def phase_table(ctx, w, abytes, fused):
             continue  # computed in the steering pass
         if merged and k == "steer_integrate":
             name = "steer_features"
+        bkey = "reorder_ranges_features" if (merged and k == "reorder_ranges") else name
         avg = v / max(nsteps, 1)
-        gbs = (abytes[name] * w.n / 1e9) / (avg / 1e3) if avg > 0 and abytes[name] > 0 else 0.0
-        out[name] = {"ms": round(avg, 5), "bytes_per_agent": round(abytes[name], 2), "GB/s": round(gbs, 1),
+        gbs = (abytes[bkey] * w.n / 1e9) / (avg / 1e3) if avg > 0 and abytes[bkey] > 0 else 0.0
+        out[name] = {"ms": round(avg, 5), "bytes_per_agent": round(abytes[bkey], 2), "GB/s": round(gbs, 1),
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": PHASE_KERNELS[name]}
     return out
 
