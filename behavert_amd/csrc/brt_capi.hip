@@ -97,6 +97,19 @@ int ensure_tmp(brt_context* ctx, size_t need_tmp) {
     return BRT_OK;
 }
 
+int ensure_xyz(brt_context* ctx, uint32_t n) {
+    const size_t need = (size_t)n * 12u;
+    if (need > ctx->xyz_bytes) {
+        if (ctx->xyz) BRT_HIP(hipFree(ctx->xyz), "hipFree(xyz)");
+        ctx->xyz = nullptr;
+        ctx->xyz_bytes = 0;
+        const size_t bytes = need + need / 4 + 4096;
+        BRT_HIP(hipMalloc((void**)&ctx->xyz, bytes), "hipMalloc(xyz)");
+        ctx->xyz_bytes = bytes;
+    }
+    return BRT_OK;
+}
+
 int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan) {
     int rc = ensure_tmp(ctx, (size_t)n * 8u);
     if (rc) return rc;
@@ -177,6 +190,7 @@ int brt_context_destroy(brt_context* ctx) {
     if (ctx->tmp) (void)hipFree(ctx->tmp);
     if (ctx->meta) (void)hipFree(ctx->meta);
     if (ctx->partials) (void)hipFree(ctx->partials);
+    if (ctx->xyz) (void)hipFree(ctx->xyz);
     delete ctx;
     return BRT_OK;
 }
@@ -807,6 +821,13 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
     const bool two_pass = !(resolve_fused(ctx, prox) && !prox->use_knn);
     const bool sorted_features = do_features && two_pass;
     if (sorted_features && (rc = ensure_tmp(ctx, (size_t)n * 16u))) return rc;
+    // with LDS lists the neighbour scan reads candidate positions from a
+    // packed x,y,z copy written by the reorder (three 16-byte loads per four
+    // candidates; the register-list scan measured no gain from it)
+    const uint32_t lds = resolve_lds_lists(ctx, prox);
+    const bool pack = !prox->use_knn && lds != 0u;
+    if (pack && (rc = ensure_xyz(ctx, n))) return rc;
+    float* xyz = pack ? ctx->xyz : nullptr;
     // phase 0: prepare (scratch + empty-cell fill) + hash + sort
     if ((rc = run_hash_sort(ctx, grid, b->pos, n, b->hash, b->cell_start, b->cell_end, s))) return rc;
     if (ev) BRT_HIP(hipEventRecord((*ev)[1], s), "hipEventRecord");
@@ -819,6 +840,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
     f.out[1] = reinterpret_cast<float4*>(b->sorted_forward);
     f.in[2] = reinterpret_cast<const float4*>(b->smooth_accel);
     f.out[2] = reinterpret_cast<float4*>(b->sorted_smooth_accel);
+    f.xyz0 = xyz;
     if (sorted_features) {
         f.count = 4;
         f.in[3] = reinterpret_cast<const float4*>(b->features);
@@ -829,7 +851,6 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
             "k_reorder_ranges");
     if (ev) BRT_HIP(hipEventRecord((*ev)[2], s), "hipEventRecord");
     const float T = brt_radius_threshold(prox->search_radius);
-    const uint32_t lds = resolve_lds_lists(ctx, prox);
     if (!two_pass) {
         // phase 2: neighbour lists + steering + integration in one pass
         BRT_HIP(brt::launch_neighbors_steer(*grid, *prox, T, *steer, dt, step,
@@ -840,7 +861,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
                                             b->lists, reinterpret_cast<float2*>(b->wander),
                                             reinterpret_cast<float4*>(b->pos), reinterpret_cast<float4*>(b->forward),
                                             reinterpret_cast<float4*>(b->smooth_accel), ctx->uniform_max_cells,
-                                            ctx->xcd_remap, lds, s),
+                                            ctx->xcd_remap, lds, s, xyz),
                 "k_prox<fused>");
         if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
         if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
@@ -848,7 +869,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
         // phase 2: neighbour lists
         BRT_HIP(brt::launch_neighbors(*grid, *prox, T, reinterpret_cast<const float4*>(b->sorted_pos),
                                       reinterpret_cast<const uint2*>(b->hash), b->cell_start, b->cell_end, n,
-                                      b->lists, ctx->uniform_max_cells, ctx->xcd_remap, lds, s),
+                                      b->lists, ctx->uniform_max_cells, ctx->xcd_remap, lds, s, xyz),
                 "k_neighbors");
         if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
         // phase 3: fused steering + integration, scattered back to agent-ID order,

@@ -295,8 +295,18 @@ __global__ void __launch_bounds__(256) k_reorder_ranges(const uint2* __restrict_
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint2 h = hash[i];
-    if (h.y < n)
-        for (uint32_t q = 0; q < f.count; ++q) f.out[q][i] = f.in[q][h.y];
+    if (h.y < n) {
+        for (uint32_t q = 0; q < f.count; ++q) {
+            const float4 v = f.in[q][h.y];
+            f.out[q][i] = v;
+            if (q == 0 && f.xyz0) {
+                float* d = f.xyz0 + 3u * (size_t)i;
+                d[0] = v.x;
+                d[1] = v.y;
+                d[2] = v.z;
+            }
+        }
+    }
     if (cell_start && h.x < num_cells) {
         if (i == 0 || hash[i - 1].x != h.x) cell_start[h.x] = i;
         if (i == n - 1 || hash[i + 1].x != h.x) cell_end[h.x] = i + 1;

@@ -20,6 +20,8 @@ struct brt_context {
     size_t meta_bytes = 0;
     float* partials = nullptr;  // adjoint: per-workgroup weight-gradient partials
     size_t partials_bytes = 0;
+    float* xyz = nullptr;       // brt_step: the sorted positions packed as x,y,z
+    size_t xyz_bytes = 0;
     bool timing = false;
     std::vector<std::array<hipEvent_t, BRT_NUM_PHASES + 1>> events;
     size_t events_used = 0;

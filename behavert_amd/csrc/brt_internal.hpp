@@ -44,6 +44,7 @@ struct ReorderArgs {
     const float4* in[kMaxReorderFeatures];
     float4* out[kMaxReorderFeatures];
     uint32_t count;
+    float* xyz0;  // optional: feature 0's x,y,z packed, 12 bytes per agent
 };
 hipError_t launch_reorder_ranges(const uint2* hash, uint32_t n, const ReorderArgs& f, uint32_t num_cells,
                                  uint32_t* cell_start, uint32_t* cell_end, hipStream_t s);
@@ -52,14 +53,17 @@ hipError_t launch_reorder_ranges(const uint2* hash, uint32_t n, const ReorderArg
 hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, float threshold,
                             const float4* spos, const uint2* hash, const uint32_t* cell_start,
                             const uint32_t* cell_end, uint32_t n, uint32_t* lists, uint32_t uniform_max_cells,
-                            uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s);
+                            uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s, const float* pxyz = nullptr);
 // neighbour lists + steering + integration in one pass (non-KNN lists only)
 hipError_t launch_neighbors_steer(const brt_grid_params& g, const brt_prox_params& p, float threshold,
                                   const brt_steer_params& sp, float dt, uint32_t step, const float4* spos,
                                   const float4* sfwd, const float4* ssa, const uint2* hash,
                                   const uint32_t* cell_start, const uint32_t* cell_end, uint32_t n, uint32_t* lists,
                                   float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
-                                  uint32_t uniform_max_cells, uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s);
+                                  uint32_t uniform_max_cells, uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s,
+                                  const float* pxyz = nullptr);
+// pxyz (optional, single-grid launches): the sorted positions packed as x,y,z,
+// 12 bytes per agent (ReorderArgs::xyz0)
 
 // opensteer
 hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t step,

@@ -133,6 +133,7 @@ struct ProxArgs {
     uint32_t xcd_remap;
     uint32_t log_rounds;  // LDS lists through the round log (n < 2^24)
     const float4* spos;
+    const float* pxyz;  // optional: the sorted positions packed as x,y,z (12 bytes each)
     const float4* sfwd;
     const float4* ssa;
     const uint2* hash;
@@ -409,6 +410,37 @@ __device__ __forceinline__ float4 bload(__amdgpu_buffer_rsrc_t r, uint32_t byte_
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t xyz_rsrc(const float* p, uint32_t n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)(n * 12u), 0x00020000);
+}
+
+// The positions of candidates j .. j+G-1.  PACKED: from the packed x,y,z copy,
+// 3G/4 16-byte loads at byte 12j (4-byte aligned; the wave's load count, not
+// its bytes, is what the address unit spends its cycles on); else one 16-byte
+// load per candidate.  Reads past the end return zeros.
+template <int G, bool PACKED>
+__device__ __forceinline__ void load_round(__amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rxyz, uint32_t j,
+                                           float4 (&q)[G]) {
+    if (PACKED) {
+        static_assert(G % 4 == 0, "packed rounds take multiples of 4 candidates");
+        constexpr int NV = 3 * G / 4;
+        float f[3 * G];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const vfloat4 v = __builtin_amdgcn_raw_buffer_load_b128(rxyz, (int)(12u * j + 16u * (uint32_t)k), 0, 0);
+            f[4 * k + 0] = v.x;
+            f[4 * k + 1] = v.y;
+            f[4 * k + 2] = v.z;
+            f[4 * k + 3] = v.w;
+        }
+#pragma unroll
+        for (int t = 0; t < G; ++t) q[t] = make_float4(f[3 * t], f[3 * t + 1], f[3 * t + 2], 0.0f);
+    } else {
+#pragma unroll
+        for (int t = 0; t < G; ++t) q[t] = bload(rpos, j * 16u + 16u * (uint32_t)t);
+    }
+}
+
 // Uniform mode: every active lane shares cell (ux,uy,uz); control flow and
 // candidate indices are wave-uniform.
 template <bool LDSL>
@@ -459,11 +491,12 @@ __device__ __forceinline__ void scan_uniform(const ProxArgs& a, const Seg& sg, L
 // Vector mode: each lane walks its own canonical stream.  The nine cell ranges
 // of one x-offset slab are loaded together (one memory latency per slab rather
 // than two per cell).
-template <bool LDSL, int G, bool SPLIT>
+template <bool LDSL, int G, bool SPLIT, bool PACKED>
 __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, Lane& L, I3 gp) {
     const brt_grid_params& g = sg.g;
     const uint32_t maxBodies = g.max_bodies_per_cell;
     const __amdgpu_buffer_rsrc_t rpos = float4_rsrc(a.spos, a.n);
+    const __amdgpu_buffer_rsrc_t rxyz = xyz_rsrc(PACKED ? a.pxyz : nullptr, PACKED ? a.n : 0u);
     BRT_CELL_LOOP
     for (int dx = -1; dx <= 1; ++dx) {
         const int cx = gp.x + dx;
@@ -492,8 +525,7 @@ __device__ __forceinline__ void scan_vector(const ProxArgs& a, const Seg& sg, La
             for (uint32_t j = s; j < lim; j += G) {
                 // candidates past lim are read (zeros past n) and not taken
                 float4 q[G];
-#pragma unroll
-                for (int t = 0; t < G; ++t) q[t] = bload(rpos, j * 16u + 16u * (uint32_t)t);
+                load_round<G, PACKED>(rpos, rxyz, j, q);
                 if (any_self)
                     test_round_any<LDSL, true, G>(a, L, j, lim - j, L.i - j, q);
                 else
@@ -563,11 +595,12 @@ __device__ __forceinline__ void flat_fetch(const ProxArgs& a, const Seg& sg, I3 
     ne = a.cell_end[key];
 }
 
-template <int G>
+template <int G, bool PACKED>
 __device__ __forceinline__ void scan_flat(const ProxArgs& a, const Seg& sg, Lane& L, I3 gp) {
     const brt_grid_params& g = sg.g;
     const uint32_t maxBodies = g.max_bodies_per_cell;
     const __amdgpu_buffer_rsrc_t rpos = float4_rsrc(a.spos, a.n);
+    const __amdgpu_buffer_rsrc_t rxyz = xyz_rsrc(PACKED ? a.pxyz : nullptr, PACKED ? a.n : 0u);
     uint32_t todo = visit_mask(sg, L, gp, a.Tskip);
     // the fetched (next) cell: ordinal nc (27: none left) and its range
     uint32_t nc = 27u, ns = BRT_EMPTY_CELL, ne = 0u;
@@ -597,8 +630,7 @@ __device__ __forceinline__ void scan_flat(const ProxArgs& a, const Seg& sg, Lane
         if (live && j < lim) {
             // candidates past lim are read (zeros past n) and not taken
             float4 q[G];
-#pragma unroll
-            for (int t = 0; t < G; ++t) q[t] = bload(rpos, j * 16u + 16u * (uint32_t)t);
+            load_round<G, PACKED>(rpos, rxyz, j, q);
             log_round<G, true>(a, L, j, lim - j, L.i - j, q);
             j += (uint32_t)G;
             if (L.cnt >= a.K) live = false;
@@ -668,15 +700,35 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
             }
             dense = __popcll(__ballot(alive && occ >= (uint32_t)BRT_PROX_DENSE_OCC)) >= BRT_PROX_DENSE_LANES;
         }
+        // with the packed positions (single-grid launches) a round's loads
+        // are 3G/4 instead of G
+        const bool packed = !BATCH && a.pxyz != nullptr;
         if (LDSL && BRT_PROX_FLAT && dense && a.log_rounds) {
-            if (alive) scan_flat<kGroupL>(a, sg, L, gp);
+            if (alive) {
+                if (packed)
+                    scan_flat<kGroupL, true>(a, sg, L, gp);
+                else
+                    scan_flat<kGroupL, false>(a, sg, L, gp);
+            }
         } else if (LDSL && BRT_PROX_FLAT_SPARSE && a.log_rounds) {
-            if (alive) scan_flat<kGroupR>(a, sg, L, gp);
+            if (alive) {
+                if (packed)
+                    scan_flat<kGroupR, true>(a, sg, L, gp);
+                else
+                    scan_flat<kGroupR, false>(a, sg, L, gp);
+            }
         } else if (alive) {
-            if (dense)
-                scan_vector<LDSL, kGroupL, BRT_PROX_SELF_SPLIT != 0>(a, sg, L, gp);
-            else
-                scan_vector<LDSL, kGroupR, false>(a, sg, L, gp);
+            if (dense) {
+                if (packed)
+                    scan_vector<LDSL, kGroupL, BRT_PROX_SELF_SPLIT != 0, true>(a, sg, L, gp);
+                else
+                    scan_vector<LDSL, kGroupL, BRT_PROX_SELF_SPLIT != 0, false>(a, sg, L, gp);
+            } else {
+                if (packed)
+                    scan_vector<LDSL, kGroupR, false, true>(a, sg, L, gp);
+                else
+                    scan_vector<LDSL, kGroupR, false, false>(a, sg, L, gp);
+            }
         }
     }
 
@@ -847,7 +899,7 @@ ProxArgs make_args(const brt_grid_params& g, const brt_prox_params& p, float thr
 hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, float threshold,
                             const float4* spos, const uint2* hash, const uint32_t* cell_start,
                             const uint32_t* cell_end, uint32_t n, uint32_t* lists, uint32_t uniform_max_cells,
-                            uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s) {
+                            uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s, const float* pxyz) {
     const uint32_t K = p.max_neighbors;
     const uint32_t W = list_stride(K);
     if (p.use_knn) {
@@ -859,6 +911,7 @@ hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, 
         return hipGetLastError();
     }
     ProxArgs a = make_args(g, p, threshold, spos, hash, cell_start, cell_end, n, lists, uniform_max_cells, xcd_remap);
+    a.pxyz = pxyz;
     const uint32_t B = prox_block(a.Ws, n);
     if (lds_lists)
         hipLaunchKernelGGL((k_prox<false, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.Ws * 4u, s, a);
@@ -872,8 +925,10 @@ hipError_t launch_neighbors_steer(const brt_grid_params& g, const brt_prox_param
                                   const float4* sfwd, const float4* ssa, const uint2* hash,
                                   const uint32_t* cell_start, const uint32_t* cell_end, uint32_t n, uint32_t* lists,
                                   float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
-                                  uint32_t uniform_max_cells, uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s) {
+                                  uint32_t uniform_max_cells, uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s,
+                                  const float* pxyz) {
     ProxArgs a = make_args(g, p, threshold, spos, hash, cell_start, cell_end, n, lists, uniform_max_cells, xcd_remap);
+    a.pxyz = pxyz;
     a.sfwd = sfwd;
     a.ssa = ssa;
     a.sp = sp;
