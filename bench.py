@@ -25,9 +25,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes_per_agent(w) -> dict:
+def algorithmic_bytes_per_agent(w, packed=False) -> dict:
     """Compulsory single-touch HBM bytes per agent for each phase of brt_step.
-    D digit passes, W list words, C/N cells per agent, F = 3 reordered float4s."""
+    D digit passes, W list words, C/N cells per agent, F = 3 reordered float4s;
+    packed: the reorder also writes the 12-byte x,y,z copy the LDS-list scan
+    reads candidates from."""
     from behavert_amd.types import list_stride
 
     n = w.n
@@ -39,8 +41,8 @@ def algorithmic_bytes_per_agent(w) -> dict:
     b = {
         # prepare (empty-cell fill 8 B/cell) + hash (R pos 16, W pair 8) + sort (R 8 hist, D x (R 8 + W 8))
         "hash_sort": 24 + (8 + 16 * D) + 8 * cpa,
-        # R pair 8 (ID + key), 3 x (gather 16 + write 16)
-        "reorder_ranges": 8 + 32 * 3,
+        # R pair 8 (ID + key), 3 x (gather 16 + write 16) [+ W packed x,y,z 12]
+        "reorder_ranges": 8 + 32 * 3 + (12 if packed else 0),
         # R own pos 16, W list 4W, R cell ranges 8 B/cell
         "neighbors": 16 + 4 * W + 8 * cpa,
         # R pos/fwd/sa 48 + list 4W + ID 4, W pos/fwd/sa 48 (neighbour gathers cache-resident)
@@ -215,8 +217,8 @@ def full_stack(steps=20, warmup=3) -> dict:
     run(warmup, steps)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    abytes = algorithmic_bytes_per_agent(w)
     fused, lds = ctx.step_shape(w.prox)
+    abytes = algorithmic_bytes_per_agent(w, packed=bool(lds) and not w.prox.use_knn)
     ph = phase_table(ctx, w, abytes, fused=fused)
     ctx.phase_timing(False)
     ctx.close()
@@ -445,7 +447,7 @@ def main():
                 fl.steer.set_targets(w.targets_at(i))
             fl.step(w.dt, i, w.feature_rate)
 
-    abytes = algorithmic_bytes_per_agent(w)
+    abytes = algorithmic_bytes_per_agent(w, packed=bool(ctx.step_shape(w.prox)[1]) and not w.prox.use_knn)
     run(0, args.warmup)
     torch.cuda.synchronize()
     # the timed steps, with HIP events between the phases on the step's stream
