@@ -38,6 +38,7 @@ struct brt_batch {
     uint32_t* d_wave_flock = nullptr;
     uint32_t* d_step = nullptr;
     void* tmp = nullptr;       // sort ping-pong (uint2[total_n])
+    float* xyz = nullptr;      // sorted positions packed as x,y,z (float[3 total_n])
     uint32_t* meta = nullptr;  // sort histograms / counters / look-back status
     size_t meta_words = 0;
     brt_batch_config cfg{};
@@ -105,6 +106,9 @@ int enqueue_step(brt_batch* b, const brt_steer_params* steer, float feature_rate
     f.out[1] = reinterpret_cast<float4*>(buf->sorted_forward);
     f.in[2] = reinterpret_cast<const float4*>(buf->smooth_accel);
     f.out[2] = reinterpret_cast<float4*>(buf->sorted_smooth_accel);
+    // the LDS-list scan reads candidates from the packed x,y,z copy
+    float* xyz = b->cfg.lds_lists ? b->xyz : nullptr;
+    f.xyz0 = xyz;
     BRT_HIP(brt::launch_reorder_ranges(reinterpret_cast<const uint2*>(buf->hash), n, f, b->total_cells,
                                        buf->cell_start, buf->cell_end, s),
             "k_reorder_ranges");
@@ -115,7 +119,7 @@ int enqueue_step(brt_batch* b, const brt_steer_params* steer, float feature_rate
                 buf->cell_start, buf->cell_end, buf->lists, reinterpret_cast<float2*>(buf->wander),
                 reinterpret_cast<float4*>(buf->pos), reinterpret_cast<float4*>(buf->forward),
                 reinterpret_cast<float4*>(buf->smooth_accel), b->cfg.prox_block, b->cfg.uniform_max_cells,
-                b->cfg.xcd_remap, b->cfg.lds_lists, s),
+                b->cfg.xcd_remap, b->cfg.lds_lists, s, xyz),
             "k_prox<batch>");
     if (buf->features && buf->features_out && feature_rate != 0.0f)
         BRT_HIP(brt::launch_features(b->prox.max_neighbors, feature_rate, n, reinterpret_cast<const uint2*>(buf->hash),
@@ -308,6 +312,7 @@ int brt_batch_create(brt_context* ctx, const brt_flock_desc* flocks, uint32_t nu
         b->meta_words = std::max(b->meta_words, brt::sort_meta_words(brt::make_sort_plan(b->total_n,
                                                                                          b->total_cells, items)));
     if ((e = hipMalloc(&b->tmp, (size_t)(b->total_n > 0 ? b->total_n : 1) * 8u)) != hipSuccess ||
+        (e = hipMalloc((void**)&b->xyz, (size_t)(b->total_n > 0 ? b->total_n : 1) * 12u)) != hipSuccess ||
         (e = hipMalloc((void**)&b->meta, b->meta_words * 4u)) != hipSuccess)
         return fail(hip_err(ctx, e, "hipMalloc(batch sort scratch)"));
     if ((e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking)) != hipSuccess)
@@ -331,6 +336,7 @@ int brt_batch_destroy(brt_batch* b) {
     if (b->d_wave_flock) (void)hipFree(b->d_wave_flock);
     if (b->d_step) (void)hipFree(b->d_step);
     if (b->tmp) (void)hipFree(b->tmp);
+    if (b->xyz) (void)hipFree(b->xyz);
     if (b->meta) (void)hipFree(b->meta);
     delete b;
     return BRT_OK;

@@ -164,6 +164,7 @@ hipError_t launch_batch_neighbors_steer(const BatchView& b, uint32_t n, const br
                                         const float4* ssa, const uint2* hash, const uint32_t* cell_start,
                                         const uint32_t* cell_end, uint32_t* lists, float2* wander, float4* pos_out,
                                         float4* fwd_out, float4* sa_out, uint32_t block, uint32_t uniform_max_cells,
-                                        uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s);
+                                        uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s,
+                                        const float* pxyz = nullptr);
 
 }  // namespace brt

@@ -700,9 +700,8 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
             }
             dense = __popcll(__ballot(alive && occ >= (uint32_t)BRT_PROX_DENSE_OCC)) >= BRT_PROX_DENSE_LANES;
         }
-        // with the packed positions (single-grid launches) a round's loads
-        // are 3G/4 instead of G
-        const bool packed = !BATCH && a.pxyz != nullptr;
+        // with the packed positions a round's loads are 3G/4 instead of G
+        const bool packed = a.pxyz != nullptr;
         if (LDSL && BRT_PROX_FLAT && dense && a.log_rounds) {
             if (alive) {
                 if (packed)
@@ -948,9 +947,10 @@ hipError_t launch_batch_neighbors_steer(const BatchView& b, uint32_t n, const br
                                         const float4* ssa, const uint2* hash, const uint32_t* cell_start,
                                         const uint32_t* cell_end, uint32_t* lists, float2* wander, float4* pos_out,
                                         float4* fwd_out, float4* sa_out, uint32_t block, uint32_t uniform_max_cells,
-                                        uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s) {
+                                        uint32_t xcd_remap, uint32_t lds_lists, hipStream_t s, const float* pxyz) {
     brt_grid_params g0{};
     ProxArgs a = make_args(g0, p, threshold, spos, hash, cell_start, cell_end, n, lists, uniform_max_cells, xcd_remap);
+    a.pxyz = pxyz;
     a.sfwd = sfwd;
     a.ssa = ssa;
     a.sp = sp;
