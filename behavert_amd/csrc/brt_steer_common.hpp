@@ -113,9 +113,13 @@ __device__ __forceinline__ V3 behaviour_force(const brt_steer_params& sp, Weight
         float wu = wsu.y + ((u1 * 2.0f - 1.0f) * walk);
         wu = wu < -1.0f ? -1.0f : (wu > 1.0f ? 1.0f : wu);
         wander[id] = make_float2(ws, wu);
-        const V3 cr = cross(v3(0.f, 1.f, 0.f), myFwd);
+        // OpenSteer's right-handed local space (LocalSpace.h:277-305) with the
+        // world up (0,1,0) as the previous up: side = normalize(forward x up),
+        // up = side x forward.  A heading along the world up has no such side;
+        // (1,0,0) stands in for it.
+        const V3 cr = cross(myFwd, v3(0.f, 1.f, 0.f));
         const V3 side = dot(cr, cr) == 0.0f ? v3(1.f, 0.f, 0.f) : normalize(cr);
-        const V3 up = cross(myFwd, side);
+        const V3 up = cross(side, myFwd);
         V3 s = side * ws + up * wu;
         force = s * sp.wander_weight + force;
     }

@@ -420,9 +420,13 @@ void steer_agent(uint32_t i, const brt_grid_params& g, const brt_steer_params& s
             wu = wu < R(-1) ? R(-1) : (wu > R(1) ? R(1) : wu);
             A.wander[2 * id] = ws;
             A.wander[2 * id + 1] = wu;
-            V3<R> c = cross(v3<R>(0, 1, 0), myFwd);
+            // OpenSteer's right-handed basis (LocalSpace.h:277-305,
+            // setUnitSideFromForwardAndUp / regenerateOrthonormalBasisUF) with
+            // the world up as the previous up: side = normalize(fwd x up),
+            // up = side x fwd; (1,0,0) when the heading is along the world up.
+            V3<R> c = cross(myFwd, v3<R>(0, 1, 0));
             V3<R> side = dot(c, c) == R(0) ? v3<R>(1, 0, 0) : normalize(c);
-            V3<R> up = cross(myFwd, side);
+            V3<R> up = cross(side, myFwd);
             V3<R> s = side * ws + up * wu;
             force = s * R(sp.wander_weight) + force;
         }
