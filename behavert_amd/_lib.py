@@ -32,6 +32,7 @@ SIGNATURES = {
     "brt_context_set_option": (_I, [_P, _I, _I]),
     "brt_context_get_option": (_I, [_P, _I, C.POINTER(_I)]),
     "brt_context_step_shape": (_I, [_P, _PP, C.POINTER(_I), C.POINTER(_I)]),
+    "brt_context_check": (_I, [_P]),
     "brt_grid_setup": (_I, [C.POINTER(_F), C.POINTER(_F), C.POINTER(_F), _U32, _GP]),
     "brt_list_stride": (_U32, [_U32]),
     "brt_radius_threshold": (_F, [_F]),
@@ -54,7 +55,8 @@ SIGNATURES = {
     "brt_bc_cluster_membership": (_I, [_P, _PP, _U32, _U32, _P, _P, _P, _P, _P, _P, _P]),
     "brt_snapshot_save": (_I, [_P, C.c_char_p, _GP, _PP, _U32, _U32, C.POINTER(StepBuffers), _P]),
     "brt_snapshot_read_info": (_I, [C.c_char_p, C.POINTER(SnapshotInfo)]),
-    "brt_snapshot_load": (_I, [_P, C.c_char_p, _U32, C.POINTER(StepBuffers), C.POINTER(SnapshotInfo), _P]),
+    "brt_snapshot_load": (_I, [_P, C.c_char_p, _U32, C.POINTER(ProxParams), C.POINTER(StepBuffers),
+                              C.POINTER(SnapshotInfo), _P]),
     "brt_osw_steer_and_slow_to_avoid_neighbors": (_I, [_P, _PP, _SP, _U32, _P, _P, _P, _P, _P, _P, _P]),
     "brt_draw_neighborhood_color": (_I, [_P, _PP, C.POINTER(_F), _U32, _U32, _P, _P, _P, _P]),
     "brt_shapes_move_towards_target": (_I, [_P, _I, C.POINTER(_F), _U32, _P, _P, _P, _P, _P, _P]),

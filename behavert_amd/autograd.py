@@ -108,14 +108,20 @@ class Rollout:
             self._step(t, self._bufs)
         return self.pos.clone(), self.fwd.clone(), self.sa.clone()
 
-    def backward(self, gpos, gfwd, gsa):
-        """dL/d(final state) -> dL/d(weights, pos0, forward0, smooth_accel0)."""
+    def backward(self, gpos, gfwd, gsa, checkpoints=None, weights=None):
+        """dL/d(final state) -> dL/d(weights, pos0, forward0, smooth_accel0).
+        checkpoints/weights: those of the forward run being differentiated
+        (default: the last forward's); the segments are replayed with them."""
+        if checkpoints is None:
+            checkpoints = self.checkpoints
+        if weights is not None:
+            self.set_weights(weights)
         cur = 0
         self.g[cur][0].copy_(gpos)
         self.g[cur][1].copy_(gfwd)
         self.g[cur][2].copy_(gsa)
         self.wgrad.zero_()
-        for (t0, p, f, a) in reversed(self.checkpoints):
+        for (t0, p, f, a) in reversed(checkpoints):
             t1 = min(t0 + self.K, self.steps)
             self.pos.copy_(p)
             self.fwd.copy_(f)
@@ -135,7 +141,12 @@ class RolloutFunction(torch.autograd.Function):
     @staticmethod
     def forward(fctx, weights, pos0, fwd0, sa0, rollout: Rollout):
         fctx.rollout = rollout
-        pos, _, _ = rollout.forward(weights.detach().cpu().tolist(), pos0.detach(), fwd0.detach(), sa0.detach())
+        w = weights.detach().cpu().tolist()
+        pos, _, _ = rollout.forward(w, pos0.detach(), fwd0.detach(), sa0.detach())
+        # this run's checkpoints and weights: the Rollout's buffers are reused
+        # by any later forward, and the backward must replay this one
+        fctx.checkpoints = rollout.checkpoints
+        fctx.weights = w
         return pos[:, :3].contiguous()
 
     @staticmethod
@@ -144,7 +155,7 @@ class RolloutFunction(torch.autograd.Function):
         gpos = torch.zeros((r.n, 4), dtype=torch.float32, device=gfinal.device)
         gpos[:, :3] = gfinal
         z = torch.zeros_like(gpos)
-        gw, gp0, gf0, ga0 = r.backward(gpos, z, z)
+        gw, gp0, gf0, ga0 = r.backward(gpos, z, z, checkpoints=fctx.checkpoints, weights=fctx.weights)
         return gw.to(gfinal.device), gp0, gf0, ga0, None
 
 

@@ -43,9 +43,15 @@ struct brt_batch {
     size_t meta_words = 0;
     brt_batch_config cfg{};
     hipStream_t cap_stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    std::vector<unsigned char> graph_key;
+    // captured steps, most recently used last; a step with the feature update
+    // alternates between two buffer sets (features in/out swapped), so two
+    // entries keep both graphs
+    struct Graph {
+        std::vector<unsigned char> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
 };
 
 namespace {
@@ -69,12 +75,18 @@ int check_config(brt_context* ctx, const brt_batch* b, const brt_batch_config* c
     return BRT_OK;
 }
 
+constexpr size_t kMaxGraphs = 2;
+
+void destroy_graph(brt_batch::Graph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+}
+
 void drop_graph(brt_batch* b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
-    b->graph_key.clear();
+    for (brt_batch::Graph& g : b->graphs) destroy_graph(g);
+    b->graphs.clear();
 }
 
 // The kernels of one batched step, in order, on stream s.
@@ -95,7 +107,7 @@ int enqueue_step(brt_batch* b, const brt_steer_params* steer, float feature_rate
     uint2* src = first;
     uint2* dst = first == A ? B : A;
     for (uint32_t q = 0; q < plan.passes; ++q) {
-        BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, b->meta, s), "k_onesweep");
+        BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, b->meta, ctx->dev_flags, s), "k_onesweep");
         std::swap(src, dst);
     }
     brt::ReorderArgs f{};
@@ -167,8 +179,14 @@ int batch_step(brt_batch* b, const brt_steer_params* steer, float feature_rate, 
     append_bytes(key, feature_rate);
     append_bytes(key, dt);
     append_bytes(key, b->cfg);
-    if (!b->exec || key != b->graph_key) {
-        drop_graph(b);
+    size_t hit = b->graphs.size();
+    for (size_t k = 0; k < b->graphs.size(); ++k)
+        if (b->graphs[k].key == key) hit = k;
+    if (hit == b->graphs.size()) {
+        if (b->graphs.size() == kMaxGraphs) {  // evict the least recently used
+            destroy_graph(b->graphs.front());
+            b->graphs.erase(b->graphs.begin());
+        }
         BRT_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         const int rc = enqueue_step(b, steer, feature_rate, dt, buf, b->cap_stream);
         hipGraph_t g = nullptr;
@@ -178,11 +196,19 @@ int batch_step(brt_batch* b, const brt_steer_params* steer, float feature_rate, 
             return rc;
         }
         if (e != hipSuccess) return hip_err(ctx, e, "hipStreamEndCapture");
-        b->graph = g;
-        BRT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-        b->graph_key = key;
+        brt_batch::Graph ng;
+        ng.graph = g;
+        ng.key = key;
+        const hipError_t ie = hipGraphInstantiate(&ng.exec, ng.graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            destroy_graph(ng);
+            return hip_err(ctx, ie, "hipGraphInstantiate");
+        }
+        b->graphs.push_back(ng);
+    } else if (hit + 1 != b->graphs.size()) {
+        std::swap(b->graphs[hit], b->graphs.back());
     }
-    BRT_HIP(hipGraphLaunch(b->exec, s), "hipGraphLaunch");
+    BRT_HIP(hipGraphLaunch(b->graphs.back().exec, s), "hipGraphLaunch");
     return BRT_OK;
 }
 
@@ -391,9 +417,10 @@ int brt_batch_autotune(brt_batch* b, const brt_steer_params* steer, float dt, co
     hipStream_t s = (hipStream_t)stream;
     const std::string key = tune_key(b, ctx->device);
     brt_batch_config best = b->cfg;
-    best.use_graph = 1;
     if (cache_path && cache_lookup(cache_path, key, &best) && check_config(ctx, b, &best) == BRT_OK) {
+        best.use_graph = b->cfg.use_graph;  // the caller's choice, as after a search
         b->cfg = best;
+        drop_graph(b);
         if (chosen) *chosen = best;
         return BRT_OK;
     }

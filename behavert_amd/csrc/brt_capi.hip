@@ -141,10 +141,24 @@ int run_hash_sort(brt_context* ctx, const brt_grid_params* g, const float* pos, 
     uint2* src = first;
     uint2* dst = first == A ? B : A;
     for (uint32_t q = 0; q < plan.passes; ++q) {
-        BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, ctx->meta, s), "k_onesweep");
+        BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, ctx->meta, ctx->dev_flags, s), "k_onesweep");
         std::swap(src, dst);
     }
     return BRT_OK;
+}
+
+int check_dev_flags(brt_context* ctx) {
+    uint32_t flags = 0;
+    BRT_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    BRT_HIP(hipMemcpy(&flags, ctx->dev_flags, sizeof(flags), hipMemcpyDeviceToHost), "hipMemcpy(device flags)");
+    if (flags == 0u) return BRT_OK;
+    BRT_HIP(hipMemset(ctx->dev_flags, 0, sizeof(uint32_t)), "hipMemset(device flags)");
+    if (flags & BRT_DEVFLAG_SORT_LOOKBACK)
+        return set_err(ctx, BRT_ERR_HIP,
+                       "a radix-sort pass gave up a look-back wait (device flag 0x%x): the sort permutation of "
+                       "a step since the last check is wrong",
+                       flags);
+    return set_err(ctx, BRT_ERR_HIP, "device flags 0x%x raised", flags);
 }
 
 void fold_events(brt_context* ctx) {
@@ -178,6 +192,12 @@ int brt_context_create(int device, brt_context** out) {
     if (hipSetDevice(device) != hipSuccess) return BRT_ERR_HIP;
     brt_context* ctx = new brt_context();
     ctx->device = device;
+    if (hipMalloc((void**)&ctx->dev_flags, 256) != hipSuccess ||
+        hipMemset(ctx->dev_flags, 0, 256) != hipSuccess) {
+        if (ctx->dev_flags) (void)hipFree(ctx->dev_flags);
+        delete ctx;
+        return BRT_ERR_HIP;
+    }
     *out = ctx;
     return BRT_OK;
 }
@@ -191,6 +211,7 @@ int brt_context_destroy(brt_context* ctx) {
     if (ctx->meta) (void)hipFree(ctx->meta);
     if (ctx->partials) (void)hipFree(ctx->partials);
     if (ctx->xyz) (void)hipFree(ctx->xyz);
+    if (ctx->dev_flags) (void)hipFree(ctx->dev_flags);
     delete ctx;
     return BRT_OK;
 }
@@ -215,6 +236,14 @@ int brt_context_set_option(brt_context* ctx, int option, int value) {
             BRT_REQUIRE(value >= -1, "lds_lists must be -1 (automatic), 0 or 1");
             ctx->lds_lists = value < 0 ? -1 : (value != 0);
             return BRT_OK;
+        case BRT_OPT_DEVICE_FLAGS: {
+            int rc = use_device(ctx);
+            if (rc) return rc;
+            const uint32_t v = (uint32_t)value;
+            BRT_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            BRT_HIP(hipMemcpy(ctx->dev_flags, &v, sizeof(v), hipMemcpyHostToDevice), "hipMemcpy(device flags)");
+            return BRT_OK;
+        }
         default:
             return set_err(ctx, BRT_ERR_INVALID_ARGUMENT, "unknown option %d", option);
     }
@@ -235,9 +264,25 @@ int brt_context_get_option(const brt_context* ctx, int option, int* value) {
         case BRT_OPT_LDS_LISTS:
             *value = ctx->lds_lists;
             return BRT_OK;
+        case BRT_OPT_DEVICE_FLAGS: {
+            brt_context* c = const_cast<brt_context*>(ctx);
+            uint32_t v = 0;
+            if (use_device(c) || hipDeviceSynchronize() != hipSuccess ||
+                hipMemcpy(&v, ctx->dev_flags, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess)
+                return BRT_ERR_HIP;
+            *value = (int)v;
+            return BRT_OK;
+        }
         default:
             return BRT_ERR_INVALID_ARGUMENT;
     }
+}
+
+int brt_context_check(brt_context* ctx) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    return check_dev_flags(ctx);
 }
 
 int brt_context_step_shape(const brt_context* ctx, const brt_prox_params* prox, int* fused, int* lds_lists) {
@@ -324,7 +369,7 @@ int brt_envgrid_sort(brt_context* ctx, uint32_t* hash, uint32_t n, uint32_t num_
     uint2* src = odd ? B : A;
     uint2* dst = odd ? A : B;
     for (uint32_t q = 0; q < plan.passes; ++q) {
-        BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, ctx->meta, s), "k_onesweep");
+        BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, ctx->meta, ctx->dev_flags, s), "k_onesweep");
         std::swap(src, dst);
     }
     return BRT_OK;
@@ -915,7 +960,7 @@ int brt_phase_times(brt_context* ctx, double* ms_out, uint32_t* steps_out) {
     fold_events(ctx);
     for (int p = 0; p < BRT_NUM_PHASES; ++p) ms_out[p] = ctx->phase_ms[p];
     if (steps_out) *steps_out = ctx->phase_steps;
-    return BRT_OK;
+    return check_dev_flags(ctx);
 }
 
 }  // extern "C"

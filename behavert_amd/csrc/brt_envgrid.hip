@@ -150,7 +150,8 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
                                                             uint32_t n, uint32_t shift, uint32_t nbits,
                                                             const uint32_t* __restrict__ ghist,
                                                             uint32_t* __restrict__ status,
-                                                            uint32_t* __restrict__ tile_counter) {
+                                                            uint32_t* __restrict__ tile_counter,
+                                                            uint32_t* __restrict__ dev_flags) {
     constexpr int TILE = kSortThreads * ITEMS;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_whist[4][kRadix];
@@ -238,9 +239,11 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
             if (flag == 0u) {
                 // A predecessor tile has started (tile IDs are taken in start order)
                 // and publishes without waiting, so this wait ends; the bound only
-                // turns an impossible stall into a wrong result instead of a hang.
+                // turns an impossible stall into a flagged wrong result instead of
+                // a hang: the sticky device flag is raised for the host to report.
                 if (++spins > (1u << 24)) {
-                    __hip_atomic_store(tile_counter + 8, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_or(dev_flags, BRT_DEVFLAG_SORT_LOOKBACK, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_SYSTEM);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
@@ -386,14 +389,16 @@ hipError_t launch_hist(const uint2* in, uint2* copy_out, uint32_t n, uint32_t* m
 }
 
 hipError_t launch_onesweep(const uint2* in, uint2* out, uint32_t n, uint32_t pass, const SortPlan& plan,
-                           uint32_t* meta, hipStream_t s) {
+                           uint32_t* meta, uint32_t* dev_flags, hipStream_t s) {
     uint32_t* status = meta + kMetaStatus + (size_t)pass * plan.tiles * kRadix;
     if (plan.items == 8u)
         hipLaunchKernelGGL(k_onesweep<8>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n, plan.shift[pass],
-                           plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass);
+                           plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass,
+                           dev_flags);
     else
         hipLaunchKernelGGL(k_onesweep<16>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n, plan.shift[pass],
-                           plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass);
+                           plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass,
+                           dev_flags);
     return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@ struct brt_context {
     size_t partials_bytes = 0;
     float* xyz = nullptr;       // brt_step: the sorted positions packed as x,y,z
     size_t xyz_bytes = 0;
+    uint32_t* dev_flags = nullptr;  // sticky device-side failure flags (BRT_DEVFLAG_*)
     bool timing = false;
     std::vector<std::array<hipEvent_t, BRT_NUM_PHASES + 1>> events;
     size_t events_used = 0;
@@ -58,6 +59,9 @@ int check_prox(brt_context* ctx, const brt_prox_params* p);
 int check_steer(brt_context* ctx, const brt_steer_params* s);
 int ensure_tmp(brt_context* ctx, size_t bytes);
 int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan);
+// synchronises the device, reads and clears the sticky flags; BRT_ERR_HIP with
+// a message when one was raised
+int check_dev_flags(brt_context* ctx);
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 

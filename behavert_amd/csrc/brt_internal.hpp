@@ -36,8 +36,9 @@ hipError_t launch_hash_hist(const brt_grid_params& g, const float4* pos, uint32_
                             uint32_t* meta, const SortPlan& plan, hipStream_t s);
 hipError_t launch_hist(const uint2* in, uint2* copy_out, uint32_t n, uint32_t* meta, const SortPlan& plan,
                        hipStream_t s);
+// dev_flags: the context's sticky device flag word (BRT_DEVFLAG_*)
 hipError_t launch_onesweep(const uint2* in, uint2* out, uint32_t n, uint32_t pass, const SortPlan& plan,
-                           uint32_t* meta, hipStream_t s);
+                           uint32_t* meta, uint32_t* dev_flags, hipStream_t s);
 
 constexpr int kMaxReorderFeatures = 8;
 struct ReorderArgs {

@@ -22,6 +22,7 @@
 // (float2[n] by ID).
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <vector>
 
 #include "brt_host.hpp"
@@ -49,6 +50,15 @@ std::vector<Section> sections(const brt_step_buffers* b, uint32_t W, uint32_t fl
     if (flags & BRT_SNAP_FEATURES) s.push_back({b->features, 16 * n});
     if (flags & BRT_SNAP_WANDER) s.push_back({b->wander, 8 * n});
     return s;
+}
+
+// bytes of the sections a file with these fields holds (sections() order)
+uint64_t payload_size(uint64_t n, uint64_t W, uint32_t flags) {
+    uint64_t b = 48 * n;
+    if (flags & BRT_SNAP_LISTS) b += 8 * n + 4 * n * W;
+    if (flags & BRT_SNAP_FEATURES) b += 16 * n;
+    if (flags & BRT_SNAP_WANDER) b += 8 * n;
+    return b;
 }
 
 uint64_t fnv1a(const unsigned char* p, size_t len, uint64_t h = 0xcbf29ce484222325ull) {
@@ -93,6 +103,21 @@ int parse_header(const unsigned char* h, brt_snapshot_info* info, std::string& e
     std::memcpy(&info->prox, h + 108, sizeof(brt_prox_params));
     info->payload_bytes = get64(h, 120);
     info->checksum = get64(h, 128);
+    if ((info->flags & ~(BRT_SNAP_LISTS | BRT_SNAP_FEATURES | BRT_SNAP_WANDER)) != 0) {
+        err = "unknown section flags";
+        return BRT_ERR_INVALID_ARGUMENT;
+    }
+    if (info->max_neighbors < 1 || info->max_neighbors > BRT_MAX_NEIGHBORS ||
+        info->list_stride != brt::list_stride(info->max_neighbors)) {
+        err = "invalid max_neighbors / list stride";
+        return BRT_ERR_INVALID_ARGUMENT;
+    }
+    // the payload size follows from n, W and the flags; a header that says
+    // otherwise is corrupt (and must not size an allocation)
+    if (info->payload_bytes != payload_size(info->n, info->list_stride, info->flags)) {
+        err = "payload size does not match n, list stride and flags";
+        return BRT_ERR_INVALID_ARGUMENT;
+    }
     return BRT_OK;
 }
 
@@ -124,6 +149,7 @@ int brt_snapshot_save(brt_context* ctx, const char* path, const brt_grid_params*
         off += s.bytes;
     }
     BRT_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize(snapshot)");
+    if ((rc = check_dev_flags(ctx))) return rc;
     unsigned char h[kHeader];
     std::memset(h, 0, sizeof(h));
     std::memcpy(h, kMagic, 8);
@@ -159,8 +185,8 @@ int brt_snapshot_read_info(const char* path, brt_snapshot_info* info) {
     return parse_header(h, info, err);
 }
 
-int brt_snapshot_load(brt_context* ctx, const char* path, uint32_t flags, const brt_step_buffers* buf,
-                      brt_snapshot_info* info_out, void* stream) {
+int brt_snapshot_load(brt_context* ctx, const char* path, uint32_t flags, const brt_prox_params* prox,
+                      const brt_step_buffers* buf, brt_snapshot_info* info_out, void* stream) {
     if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
     BRT_REQUIRE(path && buf, "NULL argument");
     FILE* f = std::fopen(path, "rb");
@@ -184,7 +210,22 @@ int brt_snapshot_load(brt_context* ctx, const char* path, uint32_t flags, const 
                        "%s: holds n = %u, flags 0x%x; asked for n = %u, flags 0x%x", path, info.n, info.flags,
                        buf->n, flags);
     }
-    std::vector<unsigned char> payload(info.payload_bytes);
+    // the destination lists hold brt_list_stride(prox->max_neighbors) slots per
+    // agent: only a file with the same K can be copied into them
+    if ((flags & BRT_SNAP_LISTS) && (!prox || prox->max_neighbors != info.max_neighbors)) {
+        std::fclose(f);
+        return set_err(ctx, BRT_ERR_INVALID_ARGUMENT,
+                       "%s: lists were saved with K = %u; the destination lists are sized for K = %u", path,
+                       info.max_neighbors, prox ? prox->max_neighbors : 0u);
+    }
+    std::vector<unsigned char> payload;
+    try {
+        payload.resize(info.payload_bytes);
+    } catch (const std::exception&) {
+        std::fclose(f);
+        return set_err(ctx, BRT_ERR_OUT_OF_MEMORY, "%s: cannot hold a payload of %llu bytes", path,
+                       (unsigned long long)info.payload_bytes);
+    }
     const bool ok = std::fread(payload.data(), 1, payload.size(), f) == payload.size();
     std::fclose(f);
     BRT_REQUIRE(ok, "%s: truncated payload", path);
@@ -221,6 +262,7 @@ int brt_snapshot_load(brt_context* ctx, const char* path, uint32_t flags, const 
         ++idx;
     }
     BRT_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize(snapshot)");
+    if ((rc = check_dev_flags(ctx))) return rc;
     if (info_out) *info_out = info;
     return BRT_OK;
 }

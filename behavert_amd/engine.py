@@ -64,6 +64,11 @@ class Context:
             msg = self._lib.brt_context_last_error(self.handle)
             raise BrtError(rc, msg.decode() if msg else "")
 
+    def check_device(self):
+        """Synchronises the device and raises BrtError if a kernel recorded a
+        failure since the last check (brt_context_check)."""
+        self.check(self._lib.brt_context_check(self.handle))
+
     # ------------------------------------------------------------ EnvGrid3D
     def calc_hash(self, grid: GridParams, pos, hash_out):
         self.check(self._lib.brt_envgrid_calc_hash(self.handle, C.byref(grid), _ptr(pos), pos.shape[0],
@@ -304,12 +309,13 @@ class Flock:
         flock of the same size; returns its header (the step it was saved at)."""
         info = SnapshotInfo()
         b = self.buffers()
-        self.ctx.check(self.ctx._lib.brt_snapshot_load(self.ctx.handle, os.fsencode(path), int(flags), C.byref(b),
+        self.ctx.check(self.ctx._lib.brt_snapshot_load(self.ctx.handle, os.fsencode(path), int(flags),
+                                                       C.byref(self.prox), C.byref(b),
                                                        C.byref(info), current_stream()))
         return info
 
     def state(self) -> dict:
-        torch.cuda.synchronize()
+        self.ctx.check_device()
         out = {"pos": self.pos.cpu().numpy(), "forward": self.forward.cpu().numpy(),
                "smooth_accel": self.smooth_accel.cpu().numpy()}
         if self.features is not None:
@@ -319,7 +325,7 @@ class Flock:
         return out
 
     def grid_outputs(self) -> dict:
-        torch.cuda.synchronize()
+        self.ctx.check_device()
         return {"hash": self.hash.cpu().numpy().view(np.uint32),
                 "cell_start": self.cell_start.cpu().numpy().view(np.uint32),
                 "cell_end": self.cell_end.cpu().numpy().view(np.uint32),

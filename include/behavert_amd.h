@@ -133,11 +133,28 @@ const char* brt_version(void);
 #define BRT_OPT_UNIFORM_MAX_CELLS 2
 #define BRT_OPT_XCD_REMAP 3
 #define BRT_OPT_LDS_LISTS 4
+/* BRT_OPT_DEVICE_FLAGS: the context's sticky device flag word (BRT_DEVFLAG_*,
+ * see brt_context_check); get synchronises the device and reads it, set writes
+ * it (for tests of the error path). */
+#define BRT_OPT_DEVICE_FLAGS 5
 int brt_context_set_option(brt_context* ctx, int option, int value);
 int brt_context_get_option(const brt_context* ctx, int option, int* value);
 /* The launch shape brt_step and brt_prox_neighbors use for these proximity
  * parameters once automatic options are resolved: fused (0/1), lds_lists (0/1). */
 int brt_context_step_shape(const brt_context* ctx, const brt_prox_params* prox, int* fused, int* lds_lists);
+
+/* Failures the kernels record on the device instead of hanging or faulting:
+ * BRT_DEVFLAG_SORT_LOOKBACK - a radix-sort pass gave up a look-back wait after
+ * its spin bound (2^24 polls); that step's sort permutation, and everything
+ * built from it, is wrong.  The flags are sticky until read.
+ * brt_context_check synchronises the device, reads and clears them, and
+ * returns BRT_ERR_HIP with a message naming the failure (BRT_OK when none was
+ * raised).  brt_phase_times, brt_snapshot_save and brt_snapshot_load, which
+ * synchronise anyway, check them too.  The reference has no counterpart (its
+ * kernels cannot report a failure; CUDA_SAFE_CALL exits on launch errors,
+ * Core/DeviceInterface.cu:106). */
+#define BRT_DEVFLAG_SORT_LOOKBACK 1u
+int brt_context_check(brt_context* ctx);
 
 /* Host-side grid set-up: EnvGrid3DPlugIn::setupGrid
  * (PlugIns/EnvGrid3D/include/EnvGrid3DPlugIn.h:478-508).  No device work. */
@@ -453,8 +470,14 @@ int brt_snapshot_save(brt_context* ctx, const char* path, const brt_grid_params*
                       const brt_step_buffers* buf, void* stream);
 /* Header only; needs no device. */
 int brt_snapshot_read_info(const char* path, brt_snapshot_info* info);
+/* prox: the proximity parameters the destination lists were sized for
+ * (brt_list_stride(prox->max_neighbors) slots per agent); with BRT_SNAP_LISTS
+ * the file's K must equal prox->max_neighbors.  May be NULL without
+ * BRT_SNAP_LISTS.  The header's payload size must follow from its n, K and
+ * flags; a file that disagrees is rejected before anything is allocated. */
 int brt_snapshot_load(brt_context* ctx, const char* path, uint32_t flags,
-                      const brt_step_buffers* buf, brt_snapshot_info* info, void* stream);
+                      const brt_prox_params* prox, const brt_step_buffers* buf,
+                      brt_snapshot_info* info, void* stream);
 
 /* Per-phase timing of brt_step with hipEvents recorded on the step's stream
  * (the reference harness's per-phase timers, BehaveRT_Flocking.cpp:171-204).

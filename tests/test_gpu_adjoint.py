@@ -305,3 +305,29 @@ def test_adjoint_rejects_unsupported_options(ctx):
     assert rc != 0
     with pytest.raises(BrtError):
         ctx.check(rc)
+
+
+def test_two_forwards_then_backward_of_the_first(ctx, orc):
+    """A Rollout reused by a second forward (other weights) before the first
+    graph's backward: the first backward replays its own checkpoints and
+    weights, so its gradients equal those of a lone forward + backward."""
+    w = small_flock(2, 2048, 50.0, K=32)
+    start, other = (1.0, 0.8, 0.6), (0.3, 1.4, 0.2)
+    pos0, f0, sa0 = dev(w.state["pos"]), dev(w.state["forward"]), dev(w.state["smooth_accel"])
+
+    def grads(ro, seq):
+        outs = []
+        for ws in seq:
+            wt = torch.tensor(ws, dtype=torch.float32, device="cuda", requires_grad=True)
+            p0 = pos0.clone().requires_grad_(True)
+            final = rollout_final_pos(ro, wt, p0, f0, sa0)
+            outs.append((wt, p0, (final ** 2).sum(dim=1).mean()))
+        wt, p0, loss = outs[0]
+        loss.backward()
+        return wt.grad.cpu().numpy(), p0.grad.cpu().numpy()
+
+    ro = Rollout(ctx, w.grid, w.prox, W.plain_steer(start), w.n, w.dt, 8, 4)
+    g_alone = grads(ro, [start])
+    g_shared = grads(ro, [start, other])
+    np.testing.assert_allclose(g_shared[0], g_alone[0], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(g_shared[1], g_alone[1], rtol=1e-4, atol=1e-7 * np.abs(g_alone[1]).max())

@@ -109,3 +109,39 @@ def test_sort_repeated_calls_reuse_scratch(ctx):
         ctx.sort(d, 40_000)
         assert first_mismatch(u32(d), h[np.lexsort((h[:, 1], h[:, 0]))]) is None
     torch.cuda.synchronize()
+
+
+def test_sort_lookback_flag_is_reported(ctx):
+    """The radix sort's bounded look-back raises a sticky device flag instead of
+    hanging; the host reports it at the next synchronising call.  The flag is
+    raised here from the host (no kernel is made to stall): brt_context_check,
+    brt_phase_times and a flock's state read then fail with BrtError, and the
+    check clears it."""
+    import ctypes as C
+
+    from behavert_amd import BrtError, lib
+    from behavert_amd.engine import Flock
+    from behavert_amd.types import DEVFLAG_SORT_LOOKBACK, OPT_DEVICE_FLAGS
+
+    L = lib()
+    v = C.c_int(-1)
+    ctx.check_device()
+    assert L.brt_context_get_option(ctx.handle, OPT_DEVICE_FLAGS, C.byref(v)) == 0 and v.value == 0
+    w = W.plain_4096(n=2048)
+    fl = Flock(ctx, w.grid, w.prox, w.steer, w.state)
+    fl.step(w.dt, 0)
+    fl.state()                                      # no flag: reads fine
+    ctx.check(L.brt_context_set_option(ctx.handle, OPT_DEVICE_FLAGS, DEVFLAG_SORT_LOOKBACK))
+    assert L.brt_context_get_option(ctx.handle, OPT_DEVICE_FLAGS, C.byref(v)) == 0
+    assert v.value == DEVFLAG_SORT_LOOKBACK
+    with pytest.raises(BrtError) as e:
+        fl.state()
+    assert e.value.code == -2 and "look-back" in str(e.value)
+    ctx.check_device()                              # cleared by the failed check
+    ctx.check(L.brt_context_set_option(ctx.handle, OPT_DEVICE_FLAGS, DEVFLAG_SORT_LOOKBACK))
+    ms = (C.c_double * 5)()
+    steps = C.c_uint32()
+    assert L.brt_phase_times(ctx.handle, ms, C.byref(steps)) == -2
+    assert L.brt_context_check(ctx.handle) == 0
+    fl.step(w.dt, 1)
+    fl.state()

@@ -3,10 +3,11 @@ flock's state as the independent numpy reader sees it, and a run resumed from a
 snapshot continues bit for bit."""
 import numpy as np
 import pytest
+import torch
 
 from behavert_amd import workloads as W
 from behavert_amd.engine import BrtError, Flock
-from behavert_amd.types import SNAP_FEATURES, SNAP_LISTS, SNAP_WANDER, SteerParams
+from behavert_amd.types import SNAP_FEATURES, SNAP_LISTS, SNAP_WANDER, ProxParams, SteerParams
 from oracle import snapshot as S
 
 pytestmark = pytest.mark.gpu
@@ -77,3 +78,26 @@ def test_snapshot_load_errors(ctx, tmp_path):
     with pytest.raises(BrtError):                      # checksum
         fl.load(str(bad))
     fl.load(str(path))                                 # the intact file still loads
+
+
+def test_snapshot_lists_need_the_same_k(ctx, tmp_path):
+    """Lists saved with K = 64 (W = 68) cannot be copied into lists sized for
+    K = 32 (W = 36), nor the other way round: the load is refused before any
+    copy, and the state sections still load without BRT_SNAP_LISTS."""
+    w = W.plain_4096(n=2048)
+    big = Flock(ctx, w.grid, ProxParams.make(64, 4.0), w.steer, w.state)
+    big.step(w.dt, 0)
+    path = tmp_path / "k64.brts"
+    big.save(str(path), 1)
+    small = Flock(ctx, w.grid, ProxParams.make(32, 4.0), w.steer, w.state)
+    with pytest.raises(BrtError) as e:
+        small.load(str(path), SNAP_LISTS)
+    assert e.value.code == -1 and "K = 64" in str(e.value)
+    small.load(str(path))                               # state only
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(small.state()["pos"].view(np.uint32), big.state()["pos"].view(np.uint32))
+    small.step(w.dt, 1)
+    p32 = tmp_path / "k32.brts"
+    small.save(str(p32), 2)
+    with pytest.raises(BrtError):
+        big.load(str(p32), SNAP_LISTS)

@@ -65,3 +65,34 @@ def test_library_rejects_bad_headers(tmp_path):
     (tmp_path / "short.brts").write_bytes(bytes(raw[:100]))
     assert lib().brt_snapshot_read_info(str(tmp_path / "short.brts").encode(), C.byref(info)) != 0
     assert lib().brt_snapshot_read_info(str(tmp_path / "missing.brts").encode(), C.byref(info)) != 0
+
+
+def test_library_rejects_inconsistent_sizes(tmp_path):
+    """A header whose payload size does not follow from its n, K and flags (a
+    corrupt or hostile file) is rejected before anything is allocated, and so
+    are an out-of-range K and a list stride that does not match it."""
+    import struct
+
+    from behavert_amd._lib import lib
+
+    w = W.plain_4096()
+    path = tmp_path / "s.brts"
+    S.write(str(path), 8, 16, list_stride(16), S.LISTS, 0, bytes(w.grid), bytes(w.prox), small_arrays(8, 20))
+    raw = bytearray(path.read_bytes())
+    info = SnapshotInfo()
+    assert lib().brt_snapshot_read_info(str(path).encode(), C.byref(info)) == 0
+    cases = {
+        "huge_payload": (120, "<Q", 1 << 62),
+        "payload_off_by_4": (120, "<Q", info.payload_bytes + 4),
+        "k_zero": (20, "<I", 0),
+        "k_too_big": (20, "<I", 253),
+        "stride_mismatch": (24, "<I", list_stride(16) + 4),
+        "unknown_flag": (28, "<I", S.LISTS | 8),
+        "n_grows": (16, "<I", 1 << 31),
+    }
+    for name, (off, fmt, val) in cases.items():
+        bad = bytearray(raw)
+        struct.pack_into(fmt, bad, off, val)
+        p = tmp_path / f"{name}.brts"
+        p.write_bytes(bytes(bad))
+        assert lib().brt_snapshot_read_info(str(p).encode(), C.byref(info)) == -1, name
