@@ -96,6 +96,8 @@ def phase_table(ctx, w, abytes, fused):
         name = "neighbors_steer" if (fused and k == "neighbors") else k
         if fused and k == "steer_integrate":
             continue
+        if k == "features" and abytes["features"] == 0:
+            continue  # no feature update ran: the phase is only the gap between two events
         if merged and k == "features":
             continue  # computed in the steering pass
         if merged and k == "steer_integrate":
@@ -106,6 +108,18 @@ def phase_table(ctx, w, abytes, fused):
         out[name] = {"ms": round(avg, 5), "bytes_per_agent": round(abytes[bkey], 2), "GB/s": round(gbs, 1),
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": PHASE_KERNELS[name]}
     return out
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(w, sample_steps: int) -> dict:
@@ -120,7 +134,7 @@ def cpu_baseline(w, sample_steps: int) -> dict:
         hf.step(w.steer, w.dt, 1 + i, w.feature_rate)
     dt = time.perf_counter() - t0
     return {"value": w.n * sample_steps / dt, "unit": "agent-updates/s", "cores": oracle.num_threads(),
-            "kind": "port",
+            "cpu_model": cpu_model(), "kind": "port",
             "sample": f"{sample_steps} steps of the {w.name} workload ({w.n} agents) after 1 untimed step, "
                       f"FP32 C++ oracle (hash, LSD radix sort, gather, ranges, 27-cell lists, steering), OpenMP",
             "ms_per_step": 1e3 * dt / sample_steps}

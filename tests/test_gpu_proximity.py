@@ -117,6 +117,21 @@ def test_dense_with_clamped_boundaries_all_scan_modes(ctx, orc, uniform_max, lis
         ctx.set_option(OPT_UNIFORM_MAX_CELLS, old)
 
 
+@pytest.mark.parametrize("K,maxb", [(32, 0xFFFFFFFF), (252, 0xFFFFFFFF), (252, 100), (64, 40)])
+def test_crowded_cells_long_streams(ctx, orc, K, maxb, list_mode):
+    """About 280 agents per cell: a cell group's stream of 27 capped ranges runs
+    to thousands of candidates, so with a large K the dense-wave scan goes past
+    the chunks it keeps in registers; maxBodiesPerCell cuts ranges (and some
+    agents' own entries) short."""
+    rng = np.random.default_rng(K + maxb % 1000)
+    grid = GridParams.direct((0.0,) * 3, (4.0,) * 3, (6,) * 3, max_bodies=maxb)
+    xyz = rng.uniform(0, 24, (60000, 3)).astype(np.float32)
+    st = W.make_state(xyz, *W.directions_and_speeds(rng, len(xyz)))
+    prox = ProxParams.make(K, 4.0)
+    o = oracle_grid_outputs(orc, grid, prox, st)
+    assert first_mismatch(gpu_lists(ctx, grid, prox, o), o["lists"]) is None
+
+
 def test_non_power_of_two_cells_offset_origin(ctx, orc, list_mode):
     """Cell size 2.5 and an offset origin: cell boxes are not exact in float, so
     the skip margins are exercised."""
