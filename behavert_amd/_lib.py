@@ -61,7 +61,8 @@ SIGNATURES = {
     "brt_draw_neighborhood_color": (_I, [_P, _PP, C.POINTER(_F), _U32, _U32, _P, _P, _P, _P]),
     "brt_shapes_move_towards_target": (_I, [_P, _I, C.POINTER(_F), _U32, _P, _P, _P, _P, _P, _P]),
     "brt_bc_cluster_hit_rate": (_I, [_P, _PP, _F, _U32, _P, _P, _P, _P, _P, _P]),
-    "brt_osw_adjoint_step": (_I, [_P, _GP, _PP, _SP, _F, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "brt_osw_adjoint_step": (_I, [_P, _GP, _PP, _SP, _F, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                  _P, _P]),
     "brt_step": (_I, [_P, _GP, _PP, _SP, _F, _F, _U32, C.POINTER(StepBuffers), _P]),
     "brt_phase_timing_enable": (_I, [_P, _I]),
     "brt_phase_times": (_I, [_P, C.POINTER(C.c_double), C.POINTER(_U32)]),

@@ -4,10 +4,11 @@ hand-written adjoint (brt_osw_adjoint_step).
 Forward: T plain steps (brt_step) from (pos0, forward0, smooth_accel0) with
 behaviour weights w; the state is checkpointed every K steps.
 Backward: for each segment, last first, the checkpoint is restored and the K
-steps are re-run with their sorted state, sort and neighbour lists recorded
-(the step is deterministic, so the records equal the forward's), then the
-adjoint kernel walks the records backwards.  The sort and the lists are thus
-held fixed as the forward built them.
+steps are re-run with their sorted state, sort, cell ranges and neighbour
+lists recorded (the step is deterministic, so the records equal the
+forward's), then the adjoint kernels walk the records backwards.  The sort and
+the lists are thus held fixed as the forward built them, and the adjoint has
+no atomics, so the gradients do not depend on the checkpoint interval.
 
 Gradients are returned for w (3 behaviour weights), pos0, forward0 (xyz = unit
 heading, w = speed) and smooth_accel0.  Differentiating with respect to an
@@ -56,27 +57,32 @@ class Rollout:
         self.r_spos, self.r_sfwd, self.r_ssa = f4(self.K, n), f4(self.K, n), f4(self.K, n)
         self.r_hash = torch.empty((self.K, n, 2), **i32)
         self.r_lists = torch.empty((self.K, n, self.W), **i32)
+        # cell ranges, rows padded to 16 bytes (the step's alignment)
+        cpad = (grid.num_cells + 3) // 4 * 4
+        self.r_cs = torch.empty((self.K, cpad), **i32)
+        self.r_ce = torch.empty((self.K, cpad), **i32)
         # adjoints (two sets, ping-pong) and the weight gradient
         self.g = [(f4(n), f4(n), f4(n)), (f4(n), f4(n), f4(n))]
         self.wgrad = torch.zeros(3, dtype=torch.float32, device=dev)
         self.checkpoints = []
         # argument blocks built once: the per-step calls only pass them on
-        self._bufs = self._buffers(self.spos, self.sfwd, self.ssa, self.hash, self.lists)
+        self._bufs = self._buffers(self.spos, self.sfwd, self.ssa, self.hash, self.lists, self.cell_start,
+                                   self.cell_end)
         self._rec_bufs = [self._buffers(self.r_spos[k], self.r_sfwd[k], self.r_ssa[k], self.r_hash[k],
-                                        self.r_lists[k]) for k in range(self.K)]
+                                        self.r_lists[k], self.r_cs[k], self.r_ce[k]) for k in range(self.K)]
         self._adj_args = {}
 
     def set_weights(self, w):
         w = [float(x) for x in w]
         self.steer.separation[0], self.steer.alignment[0], self.steer.cohesion[0] = w
 
-    def _buffers(self, spos, sfwd, ssa, hash_, lists) -> StepBuffers:
+    def _buffers(self, spos, sfwd, ssa, hash_, lists, cell_start, cell_end) -> StepBuffers:
         b = StepBuffers()
         b.n = self.n
         b.pos, b.forward, b.smooth_accel = _p(self.pos), _p(self.fwd), _p(self.sa)
         b.sorted_pos, b.sorted_forward, b.sorted_smooth_accel = _p(spos), _p(sfwd), _p(ssa)
         b.hash, b.lists = _p(hash_), _p(lists)
-        b.cell_start, b.cell_end = _p(self.cell_start), _p(self.cell_end)
+        b.cell_start, b.cell_end = _p(cell_start), _p(cell_end)
         return b
 
     def _step(self, t, bufs: StepBuffers):
@@ -90,8 +96,8 @@ class Rollout:
             go, gi = self.g[cur], self.g[1 - cur]
             args = (self.ctx.handle, C.byref(self.grid), C.byref(self.prox), C.byref(self.steer), self.dt, self.n,
                     _p(self.r_spos[k]), _p(self.r_sfwd[k]), _p(self.r_ssa[k]), _p(self.r_hash[k]),
-                    _p(self.r_lists[k]), _p(go[0]), _p(go[1]), _p(go[2]), _p(gi[0]), _p(gi[1]), _p(gi[2]),
-                    _p(self.wgrad))
+                    _p(self.r_lists[k]), _p(self.r_cs[k]), _p(self.r_ce[k]), _p(go[0]), _p(go[1]), _p(go[2]),
+                    _p(gi[0]), _p(gi[1]), _p(gi[2]), _p(self.wgrad))
             self._adj_args[key] = args
         self.ctx.check(lib().brt_osw_adjoint_step(*args, current_stream()))
 

@@ -7,13 +7,20 @@
 //   F' = adjustRawSteeringForce(F, s, f), c = truncate(F', maxF), acc = c / m,
 //   a1 = a + (acc - a) r, v0 = f s + a1 dt, v = truncate(v0, maxS),
 //   s1 = |v|, p1 = p + v dt, f1 = normalize(v).
-// Given dL/d(p1, f1, s1, a1) by agent ID this kernel recomputes the forward
-// values of agent i (same arithmetic as the forward kernels), back-propagates
-// through the integrator and the behaviour sums, writes agent i's own terms of
-// dL/d(p, f, s, a) and adds its neighbours' terms (p_j, f_j appear in i's sums)
-// into a scratch accumulator with hardware float atomics; a second kernel adds
-// the accumulator to the outputs (only the neighbour terms need atomics).  The
-// radius/angle predicates, the list membership and the world-box test are
+// Given dL/d(p1, f1, s1, a1) by agent ID, two kernels produce dL/d(p, f, s, a)
+// by agent ID, with no atomics, so the same inputs always give the same bits:
+//   k_adjoint_own (one lane per sorted agent i) recomputes agent i's forward
+//     values with the forward kernels' arithmetic, back-propagates through the
+//     integrator and its own behaviour sums, and stores by sorted index its own
+//     terms and the three vectors its neighbours' terms are made of (the
+//     separation, cohesion and alignment sums' adjoints, with flags);
+//   k_adjoint_gather (several lanes per sorted agent j) finds every agent i
+//     whose list holds j -- j's own list when that list can stand in for them,
+//     else the agents of the 27 cells around j's cell -- by repeating the list
+//     test (radius, cell range, and, for a full list, j's place in i's
+//     visiting order against i's last entry), adds i's terms for p_j and f_j
+//     in a fixed order, and writes the sums by ID.
+// The radius/angle predicates, the list membership and the world-box test are
 // piecewise constant and carry no gradient.  dL/dw for the three behaviour
 // weights is reduced per workgroup and then in a fixed order.
 // The reference has no adjoint; the forward it differentiates is the one
@@ -39,18 +46,18 @@ __device__ __forceinline__ V3 truncate_bwd(V3 x, float maxLength, V3 ybar) {
     return (ybar - u * dot(u, ybar)) * (maxLength / len);
 }
 
-__device__ __forceinline__ void atomic_add3(float4* a, V3 v) {
-    float* p = reinterpret_cast<float*>(a);
-    unsafeAtomicAdd(p + 0, v.x);
-    unsafeAtomicAdd(p + 1, v.y);
-    unsafeAtomicAdd(p + 2, v.z);
-}
+// Flags of an agent's neighbour-term vectors: which of its behaviour sums have
+// neighbour terms, whether its own cell (calcGridPos) lies in the grid, and
+// whether its list is full (count == K), and whether that cell is on a low
+// face of the grid (a coordinate 0: agents below the grid are keyed at the
+// far face, so lists there are not symmetric).
+constexpr uint32_t kAdjSep = 1u, kAdjAli = 2u, kAdjCoh = 4u, kAdjInGrid = 8u, kAdjFull = 16u, kAdjLowFace = 32u;
 
 struct AdjArgs {
     brt_grid_params g;
     brt_steer_params sp;
     float dt;
-    uint32_t W, n;
+    uint32_t K, W, n;
     const float4* spos;
     const float4* sfwd;
     const float4* ssa;
@@ -59,14 +66,11 @@ struct AdjArgs {
     const float4* gpos_out;
     const float4* gfwd_out;
     const float4* gsa_out;
-    float4* gpos_in;
-    float4* gfwd_in;
-    float4* gsa_in;
-    float4* nbr_pos;         // neighbour terms of dL/dp_j by sorted index j, accumulated with atomics
-    float4* nbr_fwd;         // neighbour terms of dL/df_j by sorted index j
-    float* partials;         // per workgroup: 3 floats
-    uint32_t* done;          // workgroups finished (reset to 0 by the last one)
-    float* weight_grad;
+    // by sorted index: 3 float4 of own terms (dL/dp, (dL/df, dL/ds), dL/da) and
+    // 3 float4 of neighbour-term vectors (S.xyz C.x | C.yz A.xy | A.z flags)
+    float4* own;
+    float4* bars;
+    float* partials;  // per workgroup: 3 floats
     uint32_t xcd_remap;
 };
 
@@ -100,7 +104,33 @@ __device__ void reduce_partials(const float* __restrict__ partials, uint32_t blo
     if (tid < 3) weight_grad[tid] = weight_grad[tid] + s_sum[tid][0];
 }
 
-__global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
+// The terms one pair (i, j) of agent i's list contributes to dL/d(off),
+// off = p_j - p_i, from i's separation and cohesion sums (anyP: one of them
+// holds the pair), and whether i's alignment sum holds it; the forward's
+// predicates, recomputed with the forward's arithmetic.
+__device__ __forceinline__ V3 pair_terms(const brt_steer_params& sp, uint32_t flags, V3 f, V3 off, V3 Sbar, V3 Cbar,
+                                         bool& anyP, bool& ali) {
+    const float d2 = dot(off, off);
+    const float dist = sqrtf(d2);
+    const float cosA = dot(f, off * (1.0f / dist));
+    const float minR = sp.sensing_min_radius;
+    V3 offbar = v3(0.f, 0.f, 0.f);
+    anyP = false;
+    if ((flags & kAdjSep) && inBoid(dist, cosA, minR, sp.separation[1], sp.separation[2])) {
+        // d/d(off) of -off/|off|^2
+        const float inv = 1.0f / d2;
+        offbar = offbar - Sbar * inv + off * (2.0f * dot(off, Sbar) * inv * inv);
+        anyP = true;
+    }
+    if ((flags & kAdjCoh) && inBoid(dist, cosA, minR, sp.cohesion[1], sp.cohesion[2])) {
+        offbar = offbar + Cbar;
+        anyP = true;
+    }
+    ali = (flags & kAdjAli) && inBoid(dist, cosA, minR, sp.alignment[1], sp.alignment[2]);
+    return offbar;
+}
+
+__global__ void __launch_bounds__(256) k_adjoint_own(AdjArgs a) {
     __shared__ float s_red[3][4];
     const uint32_t i = xcd_block(a.xcd_remap != 0) * blockDim.x + threadIdx.x;
     float wg[3] = {0.f, 0.f, 0.f};
@@ -268,20 +298,32 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
 
         // ---- backward: behaviour sums -------------------------------------------
         V3 Sbar = v3(0.f, 0.f, 0.f), Abar = Sbar, Cbar = Sbar;
+        uint32_t flags = row[0] >= a.K ? kAdjFull : 0u;
+        {
+            const I3 gpi = calcGridPos(p.x, p.y, p.z, a.g);
+            if (inGrid(gpi.x, gpi.y, gpi.z, a.g)) flags |= kAdjInGrid;
+            if (gpi.x == 0 || gpi.y == 0 || gpi.z == 0) flags |= kAdjLowFace;
+        }
         if (sp.separation[0] != 0.0f && acc.nSep > 0) {
+            flags |= kAdjSep;
             wg[0] = dot(Fbar, us);
             Sbar = normalize_bwd(xs, Fbar * sp.separation[0]) * (1.0f / (float)acc.nSep);
         }
         if (sp.alignment[0] != 0.0f && acc.nAli > 0) {
+            flags |= kAdjAli;
             wg[1] = dot(Fbar, ua);
             const V3 xbar = normalize_bwd(xa, Fbar * sp.alignment[0]);
             fbar = fbar - xbar;
             Abar = xbar * (1.0f / (float)acc.nAli);
         }
         if (sp.cohesion[0] != 0.0f && acc.nCoh > 0) {
+            flags |= kAdjCoh;
             wg[2] = dot(Fbar, uc);
             Cbar = normalize_bwd(xc, Fbar * sp.cohesion[0]) * (1.0f / (float)acc.nCoh);
         }
+        // this agent's share of the neighbour terms: p_i appears in every
+        // offset p_j - p_i of its own sums (the same per-pair terms
+        // k_adjoint_gather adds to p_j)
         if (behaviours)
             for (uint32_t t = 0; t < count; t += 4) {
                 uint32_t jj[4];
@@ -295,41 +337,26 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const uint32_t j = jj[u];
-                    if (j >= a.n) continue;
+                    if (jj[u] >= a.n) continue;
                     const V3 off = xyz(q[u]) - p;
-                    const float d2 = dot(off, off);
-                    const float dist = sqrtf(d2);
-                    const float cosA = dot(f, off * (1.0f / dist));
-                    const float minR = sp.sensing_min_radius;
-                    V3 offbar = v3(0.f, 0.f, 0.f);
-                    bool any = false;
-                    if (acc.nSep > 0 && sp.separation[0] != 0.0f &&
-                        inBoid(dist, cosA, minR, sp.separation[1], sp.separation[2])) {
-                        // d/d(off) of -off/|off|^2
-                        const float inv = 1.0f / d2;
-                        offbar = offbar - Sbar * inv + off * (2.0f * dot(off, Sbar) * inv * inv);
-                        any = true;
-                    }
-                    if (acc.nCoh > 0 && sp.cohesion[0] != 0.0f &&
-                        inBoid(dist, cosA, minR, sp.cohesion[1], sp.cohesion[2])) {
-                        offbar = offbar + Cbar;
-                        any = true;
-                    }
-                    if (any) {
-                        atomic_add3(&a.nbr_pos[j], offbar);
-                        pbar = pbar - offbar;
-                    }
-                    if (acc.nAli > 0 && sp.alignment[0] != 0.0f &&
-                        inBoid(dist, cosA, minR, sp.alignment[1], sp.alignment[2]))
-                        atomic_add3(&a.nbr_fwd[j], Abar);
+                    bool anyP, ali;
+                    const V3 offbar = pair_terms(sp, flags, f, off, Sbar, Cbar, anyP, ali);
+                    if (anyP) pbar = pbar - offbar;
                 }
             }
-        // the agent's own terms are written once (its ID appears once in the
-        // sorted hash); the neighbours' terms are added by k_adjoint_combine
-        a.gpos_in[id] = make_float4(pbar.x, pbar.y, pbar.z, 0.0f);
-        a.gfwd_in[id] = make_float4(fbar.x, fbar.y, fbar.z, sbar);
-        a.gsa_in[id] = make_float4(abar.x, abar.y, abar.z, 0.0f);
+        a.own[3u * i + 0u] = make_float4(pbar.x, pbar.y, pbar.z, 0.0f);
+        a.own[3u * i + 1u] = make_float4(fbar.x, fbar.y, fbar.z, sbar);
+        a.own[3u * i + 2u] = make_float4(abar.x, abar.y, abar.z, 0.0f);
+        a.bars[3u * i + 0u] = make_float4(Sbar.x, Sbar.y, Sbar.z, Cbar.x);
+        a.bars[3u * i + 1u] = make_float4(Cbar.y, Cbar.z, Abar.x, Abar.y);
+        a.bars[3u * i + 2u] = make_float4(Abar.z, __uint_as_float(flags), 0.0f, 0.0f);
+    } else if (i < a.n) {
+        // a hash entry without a valid ID: no terms
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint32_t k = 0; k < 3u; ++k) {
+            a.own[3u * i + k] = z;
+            a.bars[3u * i + k] = z;
+        }
     }
     // workgroup reduction of dL/dw in a fixed order
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -346,74 +373,202 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
         for (uint32_t w = 0; w < nw; ++w) v += s_red[threadIdx.x][w];
         a.partials[(size_t)blockIdx.x * 3 + threadIdx.x] = v;
     }
-    // the last workgroup to finish sums every workgroup's partials in a fixed
-    // order (no wait: it only runs once all others have published).  The
-    // partials are written by wave 0, so only wave 0 fences before the count.
-    __shared__ bool s_last;
-    if (threadIdx.x < 64u) {
-        __threadfence();
-        if (threadIdx.x == 0) s_last = atomicAdd(a.done, 1u) == gridDim.x - 1u;
-    }
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        reduce_partials(a.partials, gridDim.x, a.weight_grad);
-        if (threadIdx.x == 0) __hip_atomic_store(a.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
-__global__ void __launch_bounds__(256) k_zero3(float4* __restrict__ a, float4* __restrict__ b,
-                                               float4* __restrict__ c, float4* __restrict__ d,
-                                               uint32_t n) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        a[i] = z;
-        b[i] = z;
-        c[i] = z;
-        d[i] = z;
-        d[n + i] = z;
-    }
+// Lanes per agent of k_adjoint_gather: lane l takes entries l, l+L, ... of the
+// agent's own list (or neighbour cells l, l+L, ... of the 27 when the list
+// cannot stand in for them).
+#ifndef BRT_ADJ_GATHER_LANES
+#define BRT_ADJ_GATHER_LANES 4
+#endif
+constexpr uint32_t kGatherLanes = BRT_ADJ_GATHER_LANES;
+static_assert(kGatherLanes <= 64 && (kGatherLanes & (kGatherLanes - 1)) == 0, "lanes per agent: a power of 2");
+
+struct GatherArgs {
+    brt_grid_params g;
+    brt_steer_params sp;
+    uint32_t K, W, n;
+    float T;        // the list test's squared-radius threshold
+    uint32_t knn;   // lists built by replacement (useKnn)
+    const float4* spos;
+    const float4* sfwd;
+    const uint2* hash;
+    const uint32_t* lists;
+    const uint32_t* cell_start;
+    const uint32_t* cell_end;
+    const float4* own;
+    const float4* bars;
+    float4* gpos_in;
+    float4* gfwd_in;
+    float4* gsa_in;
+    const float* partials;
+    uint32_t partial_blocks;
+    float* weight_grad;
+};
+
+__device__ __forceinline__ I3 key_cell(uint32_t key, const brt_grid_params& g) {
+    const uint32_t gx = g.grid_size[0], gy = g.grid_size[1];
+    const uint32_t xy = key % (gx * gy);
+    return I3{(int)(xy % gx), (int)(xy / gx), (int)(key / (gx * gy))};
 }
 
-// dL/d(p, f) of each agent: its own terms plus the neighbours' terms, which
-// were accumulated by sorted index (neighbours are close in sorted order, so
-// the atomics of a wave share cache lines)
-__global__ void __launch_bounds__(256) k_adjoint_combine(float4* __restrict__ gpos_in, float4* __restrict__ gfwd_in,
-                                                         const float4* __restrict__ nbr, const uint2* __restrict__ hash,
-                                                         uint32_t n) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const uint32_t id = hash[j].y;
-    if (id >= n) return;
-    const float4 np = nbr[j], nf = nbr[n + j];
-    float4 p = gpos_in[id], f = gfwd_in[id];
-    p.x += np.x;
-    p.y += np.y;
-    p.z += np.z;
-    f.x += nf.x;
-    f.y += nf.y;
-    f.z += nf.z;
-    gpos_in[id] = p;
-    gfwd_in[id] = f;
+// The ordinal of neighbour cell c of an agent in cell gi, in the visiting
+// order (x offset outermost, z innermost).
+__device__ __forceinline__ int cell_ordinal(I3 c, I3 gi) {
+    return 9 * (c.x - gi.x + 1) + 3 * (c.y - gi.y + 1) + (c.z - gi.z + 1);
+}
+
+// Whether sorted agent j (key cell cj) is in agent i's list, given that j is
+// visible to list tests, is not i and passes the radius test against i: j must
+// lie in one of i's 27 cells (known when i lies inside the grid and was found
+// in a cell next to j's: near); then it is in the list when the list is not
+// full (every candidate that passed is), or, for a full list, when j comes no
+// later than the last entry in i's visiting order (cell ordinal, then sorted
+// index: the list is the first K that passed), or, for KNN lists
+// (replacement), when the row holds j.
+// (cj: j's key cell; for j inside the grid it is j's own cell, which from_pos
+// computes from Pj only when needed.)
+__device__ __forceinline__ bool in_list(const GatherArgs& a, uint32_t i, uint32_t flags_i, float4 Pi, uint32_t j,
+                                        I3 cj, bool from_pos, float4 Pj, bool near) {
+    const bool full = (flags_i & kAdjFull) != 0u;
+    if (near && (flags_i & kAdjInGrid) && !full) return true;
+    if (from_pos) cj = calcGridPos(Pj.x, Pj.y, Pj.z, a.g);
+    const I3 gi = calcGridPos(Pi.x, Pi.y, Pi.z, a.g);
+    if (abs(cj.x - gi.x) > 1 || abs(cj.y - gi.y) > 1 || abs(cj.z - gi.z) > 1) return false;
+    if (!full) return true;
+    const uint32_t* row = a.lists + (size_t)i * a.W;
+    if (a.knn) {
+        for (uint32_t t = 1; t <= a.K; ++t)
+            if (row[t] == j) return true;
+        return false;
+    }
+    const uint32_t last = row[a.K];
+    if (last >= a.n) return false;
+    const int oj = cell_ordinal(cj, gi), ol = cell_ordinal(key_cell(a.hash[last].x, a.g), gi);
+    return oj < ol || (oj == ol && j <= last);
+}
+
+// Adds the terms agent i's list holds for p_j and f_j (off = p_j - p_i).
+__device__ __forceinline__ void add_pair(const GatherArgs& a, uint32_t i, float4 B2, V3 off, V3& accP, V3& accF) {
+    const float4 B0 = a.bars[3u * i + 0u], B1 = a.bars[3u * i + 1u];
+    bool anyP, ali;
+    const V3 offbar = pair_terms(a.sp, __float_as_uint(B2.y), xyz(a.sfwd[i]), off, v3(B0.x, B0.y, B0.z),
+                                 v3(B0.w, B1.x, B1.y), anyP, ali);
+    if (anyP) accP = accP + offbar;
+    if (ali) accF = accF + v3(B1.z, B1.w, B2.x);
+}
+
+// The key coordinate, on one axis, of the cells whose agents can hold in their
+// lists an agent of key coordinate c: c + d for d = -1, 0, 1 inside the grid,
+// except that for c = 0 the offset -1 stands for coordinate G-1, where the
+// hash keys agents whose coordinate lies below the grid (calcGridHash's
+// unsigned clamp); -1: no such cell.
+__device__ __forceinline__ int gather_axis(int c, int d, uint32_t G) {
+    if (d < 0 && c == 0) return (int)G - 1 > 1 ? (int)G - 1 : -1;
+    const int k = c + d;
+    return (k >= 0 && k < (int)G) ? k : -1;
+}
+
+// dL/d(p_j, f_j, s_j, a_j) by agent ID: agent j's own terms plus the terms of
+// every agent i whose list holds j.  When j lies in the grid off its low faces,
+// its list is not full and maxBodiesPerCell cuts no cell, those agents are all
+// in j's own list (the radius test is symmetric: d2 has the same bits both
+// ways), so the lanes walk that list; otherwise they scan the 27 cells around
+// j's cell.  Each lane
+// sums its part in order and the lanes' sums are combined by a fixed
+// butterfly.  Workgroup 0 first adds the weight-gradient partials of
+// k_adjoint_own in a fixed order.
+__global__ void __launch_bounds__(256) k_adjoint_gather(GatherArgs a) {
+    if (blockIdx.x == 0) reduce_partials(a.partials, a.partial_blocks, a.weight_grad);
+    constexpr uint32_t L = kGatherLanes;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = t / L, l = t % L;
+    const brt_steer_params& sp = a.sp;
+    const brt_grid_params& g = a.g;
+    V3 accP = v3(0.f, 0.f, 0.f), accF = accP;
+    uint32_t id = a.n;
+    if (j < a.n) {
+        const uint2 hj = a.hash[j];
+        id = hj.y;
+        const uint32_t key = hj.x;
+        const bool behaviours = sp.separation[0] != 0.0f || sp.alignment[0] != 0.0f || sp.cohesion[0] != 0.0f;
+        const bool unlimited = g.max_bodies_per_cell == 0xFFFFFFFFu;
+        // maxBodiesPerCell hides agents past the first ones of their cell from
+        // every list test
+        const bool visible = key < g.num_cells && (unlimited || j - a.cell_start[key] < g.max_bodies_per_cell);
+        if (behaviours && id < a.n && visible) {
+            const float4 Pj = a.spos[j];
+            const V3 pj = xyz(Pj);
+            const uint32_t fj = __float_as_uint(a.bars[3u * j + 2u].y);
+            if (unlimited && (fj & kAdjInGrid) && !(fj & (kAdjFull | kAdjLowFace))) {
+                const uint32_t* row = a.lists + (size_t)j * a.W;
+                const uint32_t cnt = row[0];
+                for (uint32_t e = l; e < cnt; e += L) {
+                    const uint32_t i = row[1u + e];
+                    if (i >= a.n) continue;
+                    const float4 Pi = a.spos[i];
+                    const float4 B2 = a.bars[3u * i + 2u];
+                    if (!in_list(a, i, __float_as_uint(B2.y), Pi, j, I3{0, 0, 0}, true, Pj, true)) continue;
+                    add_pair(a, i, B2, pj - xyz(Pi), accP, accF);
+                }
+            } else {
+                const I3 cj = key_cell(key, g);
+                for (uint32_t c = l; c < 27u; c += L) {
+                    const int nx = gather_axis(cj.x, (int)(c / 9u) - 1, g.grid_size[0]);
+                    const int ny = gather_axis(cj.y, (int)((c / 3u) % 3u) - 1, g.grid_size[1]);
+                    const int nz = gather_axis(cj.z, (int)(c % 3u) - 1, g.grid_size[2]);
+                    if (nx < 0 || ny < 0 || nz < 0) continue;
+                    const bool near = abs(nx - cj.x) <= 1 && abs(ny - cj.y) <= 1 && abs(nz - cj.z) <= 1;
+                    const uint32_t nk = cellKey(nx, ny, nz, g);
+                    const uint32_t s = a.cell_start[nk];
+                    if (s == BRT_EMPTY_CELL) continue;
+                    const uint32_t e = a.cell_end[nk];
+                    for (uint32_t i = s; i < e; ++i) {
+                        if (i == j) continue;
+                        const float4 Pi = a.spos[i];
+                        // the list test of agent i for candidate j (d2 as k_prox forms it)
+                        const V3 off = pj - xyz(Pi);
+                        if (dot(off, off) > a.T) continue;
+                        const float4 B2 = a.bars[3u * i + 2u];
+                        if (!in_list(a, i, __float_as_uint(B2.y), Pi, j, cj, false, Pj, near)) continue;
+                        add_pair(a, i, B2, off, accP, accF);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t o = 1; o < L; o <<= 1) {
+        accP.x += __shfl_xor(accP.x, (int)o);
+        accP.y += __shfl_xor(accP.y, (int)o);
+        accP.z += __shfl_xor(accP.z, (int)o);
+        accF.x += __shfl_xor(accF.x, (int)o);
+        accF.y += __shfl_xor(accF.y, (int)o);
+        accF.z += __shfl_xor(accF.z, (int)o);
+    }
+    if (j < a.n && l == 0u && id < a.n) {
+        const float4 op = a.own[3u * j + 0u], of = a.own[3u * j + 1u], oa = a.own[3u * j + 2u];
+        a.gpos_in[id] = make_float4(op.x + accP.x, op.y + accP.y, op.z + accP.z, 0.0f);
+        a.gfwd_in[id] = make_float4(of.x + accF.x, of.y + accF.y, of.z + accF.z, of.w);
+        a.gsa_in[id] = oa;
+    }
 }
 
 }  // namespace
 
-hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t n,
-                          const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
-                          const uint32_t* lists, const float4* gpos_out, const float4* gfwd_out,
-                          const float4* gsa_out, float4* gpos_in, float4* gfwd_in, float4* gsa_in,
-                          float4* nbr, float* partials, uint32_t* done, float* weight_grad, uint32_t xcd_remap,
-                          hipStream_t s) {
+hipError_t launch_adjoint(const brt_grid_params& g, const brt_prox_params& p, float threshold,
+                          const brt_steer_params& sp, float dt, uint32_t n, const float4* spos, const float4* sfwd,
+                          const float4* ssa, const uint2* hash, const uint32_t* lists, const uint32_t* cell_start,
+                          const uint32_t* cell_end, const float4* gpos_out, const float4* gfwd_out,
+                          const float4* gsa_out, float4* gpos_in, float4* gfwd_in, float4* gsa_in, float4* scratch,
+                          float* partials, float* weight_grad, uint32_t xcd_remap, hipStream_t s) {
     const uint32_t blocks = (n + 255) / 256;
-    uint32_t zgrid = blocks < 2048 ? blocks : 2048;
-    hipLaunchKernelGGL(k_zero3, dim3(zgrid), dim3(256), 0, s, gpos_in, gfwd_in, gsa_in, nbr, n);
     AdjArgs a{};
     a.g = g;
     a.sp = sp;
     a.dt = dt;
-    a.W = list_stride(K);
+    a.K = p.max_neighbors;
+    a.W = list_stride(p.max_neighbors);
     a.n = n;
     a.spos = spos;
     a.sfwd = sfwd;
@@ -423,17 +578,35 @@ hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_
     a.gpos_out = gpos_out;
     a.gfwd_out = gfwd_out;
     a.gsa_out = gsa_out;
-    a.gpos_in = gpos_in;
-    a.gfwd_in = gfwd_in;
-    a.gsa_in = gsa_in;
-    a.nbr_pos = nbr;
-    a.nbr_fwd = nbr + n;
+    a.own = scratch;
+    a.bars = scratch + 3u * (size_t)n;
     a.partials = partials;
-    a.done = done;
-    a.weight_grad = weight_grad;
     a.xcd_remap = xcd_remap;
-    hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_adjoint_combine, dim3(blocks), dim3(256), 0, s, gpos_in, gfwd_in, nbr, hash, n);
+    hipLaunchKernelGGL(k_adjoint_own, dim3(blocks), dim3(256), 0, s, a);
+    GatherArgs b{};
+    b.g = g;
+    b.sp = sp;
+    b.K = p.max_neighbors;
+    b.W = a.W;
+    b.n = n;
+    b.T = threshold;
+    b.knn = p.use_knn ? 1u : 0u;
+    b.spos = spos;
+    b.sfwd = sfwd;
+    b.hash = hash;
+    b.lists = lists;
+    b.cell_start = cell_start;
+    b.cell_end = cell_end;
+    b.own = a.own;
+    b.bars = a.bars;
+    b.gpos_in = gpos_in;
+    b.gfwd_in = gfwd_in;
+    b.gsa_in = gsa_in;
+    b.partials = partials;
+    b.partial_blocks = blocks;
+    b.weight_grad = weight_grad;
+    const size_t threads = (size_t)n * kGatherLanes;
+    hipLaunchKernelGGL(k_adjoint_gather, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 

@@ -494,9 +494,10 @@ int brt_osw_steer_integrate(brt_context* ctx, const brt_grid_params* grid, const
 int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
                          const brt_steer_params* steer, float dt, uint32_t n, const float* sorted_pos,
                          const float* sorted_forward, const float* sorted_smooth_accel, const uint32_t* hash,
-                         const uint32_t* lists, const float* adj_pos_out, const float* adj_forward_out,
-                         const float* adj_smooth_accel_out, float* adj_pos_in, float* adj_forward_in,
-                         float* adj_smooth_accel_in, float* weight_grad, void* stream) {
+                         const uint32_t* lists, const uint32_t* cell_start, const uint32_t* cell_end,
+                         const float* adj_pos_out, const float* adj_forward_out, const float* adj_smooth_accel_out,
+                         float* adj_pos_in, float* adj_forward_in, float* adj_smooth_accel_in, float* weight_grad,
+                         void* stream) {
     if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
     int rc = check_grid(ctx, grid);
     if (rc) return rc;
@@ -506,8 +507,8 @@ int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid, const br
     if (steer->use_2d_projection || steer->wander_weight != 0.0f)
         return set_err(ctx, BRT_ERR_UNSUPPORTED, "the adjoint supports neither 2-D projection nor wander");
     if (n == 0) return BRT_OK;
-    BRT_REQUIRE(sorted_pos && sorted_forward && sorted_smooth_accel && hash && lists && adj_pos_out &&
-                    adj_forward_out && adj_smooth_accel_out && adj_pos_in && adj_forward_in &&
+    BRT_REQUIRE(sorted_pos && sorted_forward && sorted_smooth_accel && hash && lists && cell_start && cell_end &&
+                    adj_pos_out && adj_forward_out && adj_smooth_accel_out && adj_pos_in && adj_forward_in &&
                     adj_smooth_accel_in && weight_grad,
                 "NULL array");
     BRT_REQUIRE(aligned16(sorted_pos) && aligned16(sorted_forward) && aligned16(sorted_smooth_accel) &&
@@ -518,31 +519,30 @@ int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid, const br
                     adj_smooth_accel_in != adj_smooth_accel_out,
                 "adjoint input and output arrays must differ");
     if ((rc = use_device(ctx))) return rc;
-    // per-workgroup weight-gradient partials, then one counter word at the end
+    // per-workgroup weight-gradient partials
     const size_t need = ((size_t)(n + 255) / 256) * 3 * sizeof(float) + 256;
     if (need > ctx->partials_bytes) {
         if (ctx->partials) BRT_HIP(hipFree(ctx->partials), "hipFree(partials)");
         ctx->partials = nullptr;
         ctx->partials_bytes = 0;
         BRT_HIP(hipMalloc((void**)&ctx->partials, need), "hipMalloc(partials)");
-        BRT_HIP(hipMemset(ctx->partials, 0, need), "hipMemset(partials)");
         ctx->partials_bytes = need;
     }
-    uint32_t* done = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ctx->partials) + ctx->partials_bytes - 256);
-    // neighbour-term accumulators (2 x n float4) in the context's scratch
-    if ((rc = ensure_tmp(ctx, (size_t)n * 32u))) return rc;
-    BRT_HIP(brt::launch_adjoint(*grid, prox->max_neighbors, *steer, dt, n,
+    // each agent's own terms and neighbour-term vectors (6 x n float4) in the
+    // context's scratch
+    if ((rc = ensure_tmp(ctx, (size_t)n * 96u))) return rc;
+    BRT_HIP(brt::launch_adjoint(*grid, *prox, brt_radius_threshold(prox->search_radius), *steer, dt, n,
                                 reinterpret_cast<const float4*>(sorted_pos),
                                 reinterpret_cast<const float4*>(sorted_forward),
                                 reinterpret_cast<const float4*>(sorted_smooth_accel),
-                                reinterpret_cast<const uint2*>(hash), lists,
+                                reinterpret_cast<const uint2*>(hash), lists, cell_start, cell_end,
                                 reinterpret_cast<const float4*>(adj_pos_out),
                                 reinterpret_cast<const float4*>(adj_forward_out),
                                 reinterpret_cast<const float4*>(adj_smooth_accel_out),
                                 reinterpret_cast<float4*>(adj_pos_in), reinterpret_cast<float4*>(adj_forward_in),
                                 reinterpret_cast<float4*>(adj_smooth_accel_in),
-                                reinterpret_cast<float4*>(ctx->tmp), ctx->partials, done,
-                                weight_grad, ctx->xcd_remap, (hipStream_t)stream),
+                                reinterpret_cast<float4*>(ctx->tmp), ctx->partials, weight_grad, ctx->xcd_remap,
+                                (hipStream_t)stream),
             "k_adjoint");
     return BRT_OK;
 }

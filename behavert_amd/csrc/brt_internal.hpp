@@ -113,12 +113,12 @@ hipError_t launch_move_towards_target(int shift, const float base[3], uint32_t n
                                       float4* force_out, hipStream_t s);
 
 // adjoint
-hipError_t launch_adjoint(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t n,
-                          const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
-                          const uint32_t* lists, const float4* gpos_out, const float4* gfwd_out,
-                          const float4* gsa_out, float4* gpos_in, float4* gfwd_in, float4* gsa_in,
-                          float4* nbr, float* partials, uint32_t* done, float* weight_grad, uint32_t xcd_remap,
-                          hipStream_t s);
+hipError_t launch_adjoint(const brt_grid_params& g, const brt_prox_params& p, float threshold,
+                          const brt_steer_params& sp, float dt, uint32_t n, const float4* spos, const float4* sfwd,
+                          const float4* ssa, const uint2* hash, const uint32_t* lists, const uint32_t* cell_start,
+                          const uint32_t* cell_end, const float4* gpos_out, const float4* gfwd_out,
+                          const float4* gsa_out, float4* gpos_in, float4* gfwd_in, float4* gsa_in, float4* scratch,
+                          float* partials, float* weight_grad, uint32_t xcd_remap, hipStream_t s);
 
 inline uint32_t list_stride(uint32_t K) { return 4u * (K / 4u + 1u); }
 // Row stride of the LDS list tile of the neighbour kernels: the list stride,

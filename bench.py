@@ -192,8 +192,9 @@ def adjoint_ratio(steps=None, every=None) -> dict:
         ctx.check(L.brt_osw_adjoint_step(ctx.handle, C.byref(ro.grid), C.byref(ro.prox), C.byref(ro.steer), ro.dt,
                                          ro.n, ro.r_spos[0].data_ptr(), ro.r_sfwd[0].data_ptr(),
                                          ro.r_ssa[0].data_ptr(), ro.r_hash[0].data_ptr(), ro.r_lists[0].data_ptr(),
-                                         go[0].data_ptr(), go[1].data_ptr(), go[2].data_ptr(), gi[0].data_ptr(),
-                                         gi[1].data_ptr(), gi[2].data_ptr(), ro.wgrad.data_ptr(),
+                                         ro.r_cs[0].data_ptr(), ro.r_ce[0].data_ptr(), go[0].data_ptr(),
+                                         go[1].data_ptr(), go[2].data_ptr(), gi[0].data_ptr(), gi[1].data_ptr(),
+                                         gi[2].data_ptr(), ro.wgrad.data_ptr(),
                                          torch.cuda.current_stream().cuda_stream))
     e4.record()
     torch.cuda.synchronize()

@@ -238,21 +238,26 @@ int brt_osw_steer_integrate(brt_context* ctx, const brt_grid_params* grid,
 
 /* Adjoint (reverse mode) of brt_osw_steer_integrate with the sort and the lists
  * held fixed as the forward step built them.  No counterpart in the reference.
- * Inputs are the forward step's sorted state, hash and lists (exactly what the
- * forward step read) and dL/d(new state) by agent ID:
+ * Inputs are the forward step's sorted state, hash, lists and cell ranges
+ * (exactly what the forward step read and wrote: the lists must be those the
+ * neighbour search builds from these sorted positions, hash and ranges, as
+ * brt_step records them) and dL/d(new state) by agent ID:
  *   adj_pos_out     float4: xyz = dL/dpos
  *   adj_forward_out float4: xyz = dL/d(unit heading), w = dL/dspeed
  *   adj_smooth_accel_out float4: xyz = dL/dsmooth_accel
  * Outputs (overwritten) are dL/d(previous state) by agent ID in the same
  * layout, and dL/d(separation, alignment, cohesion weight) ADDED to the three
- * floats at weight_grad (device memory).  Neighbour contributions are summed
- * with float atomics (the last bits depend on their order).
+ * floats at weight_grad (device memory).  No atomics: the same inputs give
+ * the same bits (the neighbour terms of agent j are gathered from the agents
+ * whose lists hold j, found in the 27 cells around j's cell).
+ * The hash must be a permutation of the agent IDs (as brt_step writes it).
  * Requires use_2d_projection == 0 and wander_weight == 0. */
 int brt_osw_adjoint_step(brt_context* ctx, const brt_grid_params* grid,
                          const brt_prox_params* prox, const brt_steer_params* steer,
                          float dt, uint32_t n, const float* sorted_pos,
                          const float* sorted_forward, const float* sorted_smooth_accel,
                          const uint32_t* hash, const uint32_t* lists,
+                         const uint32_t* cell_start, const uint32_t* cell_end,
                          const float* adj_pos_out, const float* adj_forward_out,
                          const float* adj_smooth_accel_out, float* adj_pos_in,
                          float* adj_forward_in, float* adj_smooth_accel_in,

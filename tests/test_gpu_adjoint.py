@@ -39,14 +39,15 @@ def adjoint_step(ctx, grid, prox, steer, dt, spos, sfwd, ssa, h, lists, gpos_out
     (device tensors) and adds to wg (device)."""
     import ctypes as C
     from behavert_amd import lib
+    import oracle
     held = [dev(np.ascontiguousarray(a, np.float32)) for a in (spos, sfwd, ssa, gpos_out, gfwd_out, gsa_out)]
-    held += [dev(np.ascontiguousarray(h, np.uint32).view(np.int32)),
-             dev(np.ascontiguousarray(lists, np.uint32).view(np.int32))]
+    cs, ce = oracle.cell_ranges(np.ascontiguousarray(h, np.uint32), grid.num_cells)
+    held += [dev(np.ascontiguousarray(a, np.uint32).view(np.int32)) for a in (h, lists, cs, ce)]
     n = len(h)
     gi = [empty_f32(n, 4) for _ in range(3)]
     P = [t.data_ptr() for t in held]
     ctx.check(lib().brt_osw_adjoint_step(ctx.handle, C.byref(grid), C.byref(prox), C.byref(steer), dt, n,
-                                         P[0], P[1], P[2], P[6], P[7], P[3], P[4], P[5],
+                                         P[0], P[1], P[2], P[6], P[7], P[8], P[9], P[3], P[4], P[5],
                                          gi[0].data_ptr(), gi[1].data_ptr(), gi[2].data_ptr(), wg.data_ptr(),
                                          torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
@@ -129,13 +130,17 @@ def test_adjoint_chain_on_fp64_trajectory(ctx, orc, T, tol):
     w0 = np.array([1.0, 0.8, 0.6], np.float32).astype(np.float64)
     steer0 = W.plain_steer(tuple(w0))
     s0 = velocity_state(st, v64)
-    st32 = {k: s0[k].astype(np.float32) for k in s0}
-    _, recs32 = record_forward(orc, w, steer0, T, state=st32)
     s = {k: v.copy() for k, v in s0.items()}
     recs, lin = [], []
-    for t, (h, L, _, _) in enumerate(recs32):
+    for t in range(T):
+        # the sort and the lists of the state being linearised at (the adjoint
+        # takes lists that are those of the sorted positions it is given)
+        pos32 = s["pos"].astype(np.float32)
+        h = orc.sort_radix(orc.calc_hash(w.grid, pos32), w.grid.num_cells)
+        cs, ce = orc.cell_ranges(h, w.grid.num_cells)
         ids = h[:, 1].astype(np.int64)
         spos, sfwd, ssa = (s[k][ids].astype(np.float32) for k in ("pos", "forward", "smooth_accel"))
+        L = orc.neighbors(w.grid, w.prox, spos, h, cs, ce)
         recs.append((h, L, spos, sfwd))
         lin.append((spos, sfwd, ssa))
         out = orc.step_f64_fixed(w.grid, w.prox, steer0, w.dt, t, s, h, L, pred=(spos, sfwd))
@@ -277,15 +282,59 @@ def test_rollout_gradient_agreement_by_horizon(ctx, orc, record_property):
 
 
 def test_checkpoint_interval_does_not_change_gradients(ctx, orc):
-    """The replayed segments equal the forward, so the gradients agree up to the
-    order of the adjoint's float atomics: a component summed from terms that
-    cancel can differ by a few ulp of its largest term, so the absolute
-    tolerance is 1e-7 of the largest component (a wrong replay moves
-    components by far more)."""
+    """The replayed segments equal the forward and the adjoint has no atomics,
+    so the gradients are bit-identical for every checkpoint interval."""
     res = [rollout_case(ctx, orc, 2048, 50.0, 12, every) for every in (1, 5, 12)]
     for _, _, gw, gv, _, _ in res[1:]:
-        np.testing.assert_allclose(gw, res[0][2], rtol=1e-5, atol=1e-9)
-        np.testing.assert_allclose(gv, res[0][3], rtol=1e-4, atol=1e-7 * np.abs(res[0][3]).max())
+        assert np.array_equal(gw.view(np.uint32), res[0][2].view(np.uint32))
+        assert np.array_equal(gv.view(np.uint32), res[0][3].view(np.uint32))
+
+
+def test_adjoint_step_is_deterministic_and_matches_a_scatter_of_the_pair_terms(ctx, orc):
+    """Two calls on the same inputs give the same bits; and with one agent's
+    cotangent set, the neighbour terms land on exactly the agents of its
+    list (the gather finds every list that holds an agent, including full
+    lists cut by K and lists cut by maxBodiesPerCell)."""
+    rng = np.random.default_rng(12)
+    for maxb, K, side in ((0xFFFFFFFF, 32, 30.0), (6, 8, 14.0)):
+        grid = W.cube_grid(side, max_bodies=maxb)
+        n = 3000
+        # agents below and above the grid on every axis: the hash keys those
+        # below at the far face (unsigned clamp), and their lists still hold
+        # agents of the near face
+        st = W.make_state(rng.uniform(-3.0, side + 3.0, (n, 3)).astype(np.float32),
+                          *W.directions_and_speeds(rng, n))
+        prox = ProxParams.make(K, 4.0)
+        w = W.Workload("det", n, side, grid, prox, W.plain_steer(), 0.05, 1, st)
+        _, recs = record_forward(orc, w, w.steer, 1)
+        h, L, spos, sfwd = recs[0]
+        ids = h[:, 1].astype(np.int64)
+        ssa = st["smooth_accel"][ids]
+        if K == 8:
+            assert (L[:, 0] == K).mean() > 0.3, "the case must have full lists"
+        cot = [rng.standard_normal((n, 4)).astype(np.float32) for _ in range(3)]
+        outs = []
+        for _ in range(2):
+            wg = torch.zeros(3, dtype=torch.float32, device="cuda")
+            gi = adjoint_step(ctx, grid, prox, w.steer, w.dt, spos, sfwd, ssa, h, L, *cot, wg)
+            outs.append([f32(x) for x in gi] + [f32(wg)])
+        for a, b in zip(*outs):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        # one agent's cotangent: the neighbour terms reach exactly its list
+        # (a full list, a list of an agent below the grid, the longest list)
+        below = np.nonzero((spos[:, :3] < 0).any(axis=1) & (L[:, 0] > 0))[0]
+        picks = {int(np.argmax(L[:, 0])), int(below[0])}
+        if K == 8:
+            picks.add(int(np.nonzero(L[:, 0] == K)[0][-1]))
+        for i in sorted(picks):
+            one = [np.zeros((n, 4), np.float32) for _ in range(3)]
+            one[0][h[i, 1], :3] = 1.0
+            wg = torch.zeros(3, dtype=torch.float32, device="cuda")
+            gi = adjoint_step(ctx, grid, prox, w.steer, w.dt, spos, sfwd, ssa, h, L, *one, wg)
+            gp = f32(gi[0])
+            touched = set(np.nonzero(np.abs(gp[:, :3]).sum(1) > 0)[0].tolist())
+            expect = {int(h[j, 1]) for j in L[i, 1:1 + L[i, 0]]} | {int(h[i, 1])}
+            assert touched == expect, (i, sorted(touched - expect), sorted(expect - touched))
 
 
 def test_adjoint_rejects_unsupported_options(ctx):
@@ -301,7 +350,8 @@ def test_adjoint_rejects_unsupported_options(ctx):
     rc = lib().brt_osw_adjoint_step(ctx.handle, C.byref(w.grid), C.byref(w.prox), C.byref(sp), w.dt, w.n,
                                     z.data_ptr(), z.data_ptr(), z.data_ptr(), z.data_ptr(), z.data_ptr(),
                                     z.data_ptr(), z.data_ptr(), z.data_ptr(), z.data_ptr(), z.data_ptr(),
-                                    z.data_ptr(), z.data_ptr(), torch.cuda.current_stream().cuda_stream)
+                                    z.data_ptr(), z.data_ptr(), z.data_ptr(), z.data_ptr(),
+                                    torch.cuda.current_stream().cuda_stream)
     assert rc != 0
     with pytest.raises(BrtError):
         ctx.check(rc)
