@@ -211,14 +211,19 @@ def rollout_case(ctx, orc, n, side, steps, every, seed=0, w=None, offset=0.0):
     loss = ((final - target) ** 2).sum(dim=1).mean()
     loss.backward()
     w.extra["fwd0_used"] = fwd0.detach().cpu().numpy()
+    w.extra["final"] = final.detach().cpu().numpy()
     return w, ro, weights.grad.detach().cpu().numpy(), v0.grad.detach().cpu().numpy(), target.cpu().numpy(), \
         v0.detach().cpu().numpy()
 
 
-def rollout_fd_errors(ctx, orc, w, steps, every, coords=16, offset=0.0):
+def rollout_fd_errors(ctx, orc, w, steps, every, coords=16, offset=0.0, linearised=False):
     """GPU autograd gradients and FP64 central differences of the fixed-decision
     rollout along the FP32 forward's sorts and lists, for dL/dweights and for
-    dL/dv0 on `coords` coordinates."""
+    dL/dv0 on `coords` coordinates.  linearised: the differences are those of
+    the loss linearised at the FP32 forward, sum(c * final) with the cotangent
+    c = 2 (final32 - target) / n fixed -- the gradient the adjoint computes is
+    J^T c, so this leaves out the FP32 rounding of the residual (final -
+    target), which for a small residual dominates a per-agent gradient."""
     w, ro, gw, gv, target, v0 = rollout_case(ctx, orc, 0, 0.0, steps, every, w=w, offset=offset)
     n = w.n
     st = dict(w.state)
@@ -226,10 +231,13 @@ def rollout_fd_errors(ctx, orc, w, steps, every, coords=16, offset=0.0):
     _, recs = record_forward(orc, w, W.plain_steer((1.0, 0.8, 0.6)), steps, state=st)
     v64 = v0.astype(np.float64)
     tgt = target.astype(np.float64)
+    cot = 2.0 * (w.extra["final"].astype(np.float64) - tgt) / n
 
     def loss(v, weights):
         out = orc.rollout_f64_fixed(w.grid, w.prox, W.plain_steer(tuple(weights)), w.dt, recs,
                                     velocity_state(st, v))
+        if linearised:
+            return (out["pos"][:, :3] * cot).sum()
         return ((out["pos"][:, :3] - tgt) ** 2).sum(axis=1).mean()
 
     w0 = np.array([1.0, 0.8, 0.6], np.float32).astype(np.float64)
