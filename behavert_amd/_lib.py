@@ -64,6 +64,7 @@ SIGNATURES = {
     "brt_osw_adjoint_step": (_I, [_P, _GP, _PP, _SP, _F, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                   _P, _P]),
     "brt_step": (_I, [_P, _GP, _PP, _SP, _F, _F, _U32, C.POINTER(StepBuffers), _P]),
+    "brt_step_noalloc": (_I, [_P, _GP, _PP, _SP, _F, _F, _U32, C.POINTER(StepBuffers), _P]),
     "brt_phase_timing_enable": (_I, [_P, _I]),
     "brt_phase_times": (_I, [_P, C.POINTER(C.c_double), C.POINTER(_U32)]),
     "brt_batch_create": (_I, [_P, C.POINTER(FlockDesc), _U32, _PP, C.POINTER(_P)]),

@@ -126,22 +126,29 @@ int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan) {
 }
 
 // hash (+ histograms) then the onesweep passes; the sorted pairs end in `hash`.
+// meta: sort metadata (sort_meta_words), tmp: ping-pong uint2[n]; both given
+// (meta == nullptr: the context's, allocated as needed).
 int run_hash_sort(brt_context* ctx, const brt_grid_params* g, const float* pos, uint32_t n, uint32_t* hash,
-                  uint32_t* cell_start, uint32_t* cell_end, hipStream_t s) {
+                  uint32_t* cell_start, uint32_t* cell_end, hipStream_t s, uint32_t* meta = nullptr,
+                  void* tmp = nullptr) {
     brt::SortPlan plan = brt::make_sort_plan(n, g->num_cells);
-    int rc = ensure_scratch(ctx, n, plan);
-    if (rc) return rc;
-    BRT_HIP(brt::launch_prepare(ctx->meta, brt::sort_meta_words(plan), cell_start, cell_end, g->num_cells, s),
+    if (!meta) {
+        int rc = ensure_scratch(ctx, n, plan);
+        if (rc) return rc;
+        meta = ctx->meta;
+        tmp = ctx->tmp;
+    }
+    BRT_HIP(brt::launch_prepare(meta, brt::sort_meta_words(plan), cell_start, cell_end, g->num_cells, s),
             "k_prepare");
     uint2* A = reinterpret_cast<uint2*>(hash);
-    uint2* B = reinterpret_cast<uint2*>(ctx->tmp);
+    uint2* B = reinterpret_cast<uint2*>(tmp);
     uint2* first = (plan.passes & 1u) ? B : A;
-    BRT_HIP(brt::launch_hash_hist(*g, reinterpret_cast<const float4*>(pos), n, first, ctx->meta, plan, s),
+    BRT_HIP(brt::launch_hash_hist(*g, reinterpret_cast<const float4*>(pos), n, first, meta, plan, s),
             "k_hash_hist");
     uint2* src = first;
     uint2* dst = first == A ? B : A;
     for (uint32_t q = 0; q < plan.passes; ++q) {
-        BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, ctx->meta, ctx->dev_flags, s), "k_onesweep");
+        BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, meta, ctx->dev_flags, s), "k_onesweep");
         std::swap(src, dst);
     }
     return BRT_OK;
@@ -183,6 +190,20 @@ extern "C" {
 
 const char* brt_version(void) { return "behavert_amd 0.1 (gfx950)"; }
 
+// ------------------------------------------------------------ no-allocation step
+// The scratch of brt_step_noalloc: one file-scope device array (each GPU holds
+// its own copy, at its own address, for the life of the process), laid out as
+// sort metadata, a 16-byte-per-agent buffer and the packed positions, sized for
+// BRT_NOALLOC_MAX_AGENTS agents (about 30 MB).  Host code reaches it through
+// the address hipGetSymbolAddress returns on each device, kept in the context.
+constexpr size_t kNoAllocMetaWords = 1088 + 4 * ((BRT_NOALLOC_MAX_AGENTS + 2047) / 2048) * 256;
+constexpr size_t kNoAllocMetaOffset = 0;
+constexpr size_t kNoAllocTmpOffset = (kNoAllocMetaOffset + kNoAllocMetaWords * 4 + 255) / 256 * 256;
+constexpr size_t kNoAllocXyzOffset = (kNoAllocTmpOffset + (size_t)BRT_NOALLOC_MAX_AGENTS * 16 + 255) / 256 * 256;
+constexpr size_t kNoAllocBytes = (kNoAllocXyzOffset + (size_t)BRT_NOALLOC_MAX_AGENTS * 12 + 255) / 256 * 256;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 int brt_context_create(int device, brt_context** out) {
     if (!out) return BRT_ERR_INVALID_ARGUMENT;
     *out = nullptr;
@@ -195,6 +216,12 @@ int brt_context_create(int device, brt_context** out) {
     if (hipMalloc((void**)&ctx->dev_flags, 256) != hipSuccess ||
         hipMemset(ctx->dev_flags, 0, 256) != hipSuccess) {
         if (ctx->dev_flags) (void)hipFree(ctx->dev_flags);
+        delete ctx;
+        return BRT_ERR_HIP;
+    }
+    // this device's copy of the no-allocation scratch (device current)
+    if (hipGetSymbolAddress(&ctx->scratch, HIP_SYMBOL(scratch)) != hipSuccess) {
+        (void)hipFree(ctx->dev_flags);
         delete ctx;
         return BRT_ERR_HIP;
     }
@@ -822,9 +849,21 @@ int brt_bc_feature_update(brt_context* ctx, const brt_prox_params* prox, float r
     return BRT_OK;
 }
 
-int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
-             const brt_steer_params* steer, float feature_rate, float dt, uint32_t step,
-             const brt_step_buffers* b, void* stream) {
+namespace {
+
+// Scratch of one step given by the caller of step_core instead of the
+// context's (the no-allocation entry point): sort metadata, a buffer of 16
+// bytes per agent (sort ping-pong, then the sorted features), the packed x,y,z
+// positions (12 bytes per agent).
+struct StepScratch {
+    uint32_t* meta;
+    void* tmp;
+    float* xyz;
+};
+
+int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
+              const brt_steer_params* steer, float feature_rate, float dt, uint32_t step,
+              const brt_step_buffers* b, void* stream, const StepScratch* given) {
     if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
     int rc = check_grid(ctx, grid);
     if (rc) return rc;
@@ -850,7 +889,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
     hipStream_t s = (hipStream_t)stream;
 
     std::array<hipEvent_t, BRT_NUM_PHASES + 1>* ev = nullptr;
-    if (ctx->timing) {
+    if (ctx->timing && !given) {
         if (ctx->events_used == 4096) fold_events(ctx);
         if (ctx->events_used == ctx->events.size()) {
             std::array<hipEvent_t, BRT_NUM_PHASES + 1> a;
@@ -865,16 +904,24 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
     // steering pass reads the neighbours' features by sorted index
     const bool two_pass = !(resolve_fused(ctx, prox) && !prox->use_knn);
     const bool sorted_features = do_features && two_pass;
-    if (sorted_features && (rc = ensure_tmp(ctx, (size_t)n * 16u))) return rc;
+    if (!given && sorted_features && (rc = ensure_tmp(ctx, (size_t)n * 16u))) return rc;
     // with LDS lists the neighbour scan reads candidate positions from a
     // packed x,y,z copy written by the reorder (three 16-byte loads per four
     // candidates; the register-list scan measured no gain from it)
     const uint32_t lds = resolve_lds_lists(ctx, prox);
     const bool pack = !prox->use_knn && lds != 0u;
-    if (pack && (rc = ensure_xyz(ctx, n))) return rc;
-    float* xyz = pack ? ctx->xyz : nullptr;
-    // phase 0: prepare (scratch + empty-cell fill) + hash + sort
-    if ((rc = run_hash_sort(ctx, grid, b->pos, n, b->hash, b->cell_start, b->cell_end, s))) return rc;
+    if (!given && pack && (rc = ensure_xyz(ctx, n))) return rc;
+    float* xyz = pack ? (given ? given->xyz : ctx->xyz) : nullptr;
+    void* tmp = nullptr;
+    if (given) {
+        tmp = given->tmp;
+        if ((rc = run_hash_sort(ctx, grid, b->pos, n, b->hash, b->cell_start, b->cell_end, s, given->meta, tmp)))
+            return rc;
+    } else {
+        // phase 0: prepare (scratch + empty-cell fill) + hash + sort
+        if ((rc = run_hash_sort(ctx, grid, b->pos, n, b->hash, b->cell_start, b->cell_end, s))) return rc;
+        tmp = ctx->tmp;
+    }
     if (ev) BRT_HIP(hipEventRecord((*ev)[1], s), "hipEventRecord");
     // phase 1: reorder pos/forward/smooth_accel + cell ranges
     brt::ReorderArgs f{};
@@ -889,7 +936,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
     if (sorted_features) {
         f.count = 4;
         f.in[3] = reinterpret_cast<const float4*>(b->features);
-        f.out[3] = reinterpret_cast<float4*>(ctx->tmp);
+        f.out[3] = reinterpret_cast<float4*>(tmp);
     }
     BRT_HIP(brt::launch_reorder_ranges(reinterpret_cast<const uint2*>(b->hash), n, f, grid->num_cells, b->cell_start,
                                        b->cell_end, s),
@@ -929,7 +976,7 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
                                   nullptr, ctx->xcd_remap, s, feature_rate,
                                   do_features ? reinterpret_cast<const float4*>(b->features) : nullptr,
                                   do_features ? reinterpret_cast<float4*>(b->features_out) : nullptr,
-                                  sorted_features ? reinterpret_cast<const float4*>(ctx->tmp) : nullptr),
+                                  sorted_features ? reinterpret_cast<const float4*>(tmp) : nullptr),
                 "k_steer");
         if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
         do_features = false;
@@ -942,6 +989,35 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_param
                 "k_features");
     if (ev) BRT_HIP(hipEventRecord((*ev)[5], s), "hipEventRecord");
     return BRT_OK;
+}
+
+}  // namespace
+
+int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
+             const brt_steer_params* steer, float feature_rate, float dt, uint32_t step,
+             const brt_step_buffers* b, void* stream) {
+    return step_core(ctx, grid, prox, steer, feature_rate, dt, step, b, stream, nullptr);
+}
+
+int brt_step_noalloc(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
+                     const brt_steer_params* steer, float feature_rate, float dt, uint32_t step,
+                     const brt_step_buffers* b, void* stream) {
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    BRT_REQUIRE(b != nullptr, "buffers are NULL");
+    BRT_REQUIRE(grid != nullptr, "grid params are NULL");
+    BRT_REQUIRE(b->n <= BRT_NOALLOC_MAX_AGENTS, "n = %u exceeds the no-allocation step's maximum of %u agents",
+                b->n, (unsigned)BRT_NOALLOC_MAX_AGENTS);
+    const brt::SortPlan plan = brt::make_sort_plan(b->n, grid->num_cells);
+    BRT_REQUIRE(brt::sort_meta_words(plan) <= kNoAllocMetaWords,
+                "the sort metadata of this step (%zu words) exceeds the no-allocation scratch (%zu words)",
+                brt::sort_meta_words(plan), kNoAllocMetaWords);
+    BRT_REQUIRE(ctx->scratch != nullptr, "the device scratch of the no-allocation step is not set up");
+    char* base = reinterpret_cast<char*>(ctx->scratch);
+    StepScratch sc;
+    sc.meta = reinterpret_cast<uint32_t*>(base + kNoAllocMetaOffset);
+    sc.tmp = base + kNoAllocTmpOffset;
+    sc.xyz = reinterpret_cast<float*>(base + kNoAllocXyzOffset);
+    return step_core(ctx, grid, prox, steer, feature_rate, dt, step, b, stream, &sc);
 }
 
 int brt_phase_timing_enable(brt_context* ctx, int enable) {

@@ -23,6 +23,7 @@ struct brt_context {
     float* xyz = nullptr;       // brt_step: the sorted positions packed as x,y,z
     size_t xyz_bytes = 0;
     uint32_t* dev_flags = nullptr;  // sticky device-side failure flags (BRT_DEVFLAG_*)
+    void* scratch = nullptr;        // this device's no-allocation step scratch (a static device array)
     bool timing = false;
     std::vector<std::array<hipEvent_t, BRT_NUM_PHASES + 1>> events;
     size_t events_used = 0;

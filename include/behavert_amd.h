@@ -449,6 +449,24 @@ int brt_step(brt_context* ctx, const brt_grid_params* grid,
              float feature_rate, float dt, uint32_t step,
              const brt_step_buffers* buf, void* stream);
 
+/* brt_step with no allocation: the same step, bit for bit, with its scratch
+ * (sort metadata, the sort ping-pong / sorted features, the packed positions)
+ * in a static device array of the library instead of the context's buffers,
+ * so the call makes no hipMalloc/hipFree (and records no phase-timing
+ * events, which brt_phase_timing_enable would create).  No counterpart in the
+ * reference:
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid allocation).  Limit: n <= BRT_NOALLOC_MAX_AGENTS
+ * (the array holds about 30 MB on each GPU the library runs on, for the life
+ * of the process); larger n returns BRT_ERR_INVALID_ARGUMENT.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other host threads, other contexts) race. */
+#define BRT_NOALLOC_MAX_AGENTS (1u << 20)
+int brt_step_noalloc(brt_context* ctx, const brt_grid_params* grid,
+                     const brt_prox_params* prox, const brt_steer_params* steer,
+                     float feature_rate, float dt, uint32_t step,
+                     const brt_step_buffers* b, void* stream);
+
 /* ---------------------------------------------------------------- state snapshots
  * Binary snapshot of a flock (no counterpart in the reference, which keeps no
  * state on disk; its host round trips are DeviceArrayWrapper::copyArrayFromDevice
