@@ -239,6 +239,11 @@ int brt_context_destroy(brt_context* ctx) {
     if (ctx->partials) (void)hipFree(ctx->partials);
     if (ctx->xyz) (void)hipFree(ctx->xyz);
     if (ctx->dev_flags) (void)hipFree(ctx->dev_flags);
+    for (auto& g : ctx->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
     delete ctx;
     return BRT_OK;
 }
@@ -262,6 +267,9 @@ int brt_context_set_option(brt_context* ctx, int option, int value) {
         case BRT_OPT_LDS_LISTS:
             BRT_REQUIRE(value >= -1, "lds_lists must be -1 (automatic), 0 or 1");
             ctx->lds_lists = value < 0 ? -1 : (value != 0);
+            return BRT_OK;
+        case BRT_OPT_STEP_GRAPH:
+            ctx->step_graph = value != 0;
             return BRT_OK;
         case BRT_OPT_DEVICE_FLAGS: {
             int rc = use_device(ctx);
@@ -290,6 +298,9 @@ int brt_context_get_option(const brt_context* ctx, int option, int* value) {
             return BRT_OK;
         case BRT_OPT_LDS_LISTS:
             *value = ctx->lds_lists;
+            return BRT_OK;
+        case BRT_OPT_STEP_GRAPH:
+            *value = ctx->step_graph;
             return BRT_OK;
         case BRT_OPT_DEVICE_FLAGS: {
             brt_context* c = const_cast<brt_context*>(ctx);
@@ -993,10 +1004,120 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
 
 }  // namespace
 
+namespace {
+
+void append_raw(std::vector<unsigned char>& k, const void* v, size_t bytes) {
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(v);
+    k.insert(k.end(), p, p + bytes);
+}
+#define append_key(k, v) append_raw((k), &(v), sizeof(v))
+
+void destroy_step_graph(brt_context::StepGraph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+}
+
+// The key of a step: everything the captured kernels take by value or by
+// address, apart from the context's scratch (checked separately).
+std::vector<unsigned char> step_key(const brt_context* ctx, const brt_grid_params* grid,
+                                    const brt_prox_params* prox, const brt_steer_params* steer, float feature_rate,
+                                    float dt, uint32_t step, const brt_step_buffers* b) {
+    std::vector<unsigned char> key;
+    append_key(key, *b);
+    append_key(key, *grid);
+    append_key(key, *prox);
+    append_key(key, *steer);
+    append_key(key, feature_rate);
+    append_key(key, dt);
+    const uint32_t wstep = steer->wander_weight != 0.0f ? step : 0u;
+    append_key(key, wstep);
+    append_key(key, ctx->fused_step);
+    append_key(key, ctx->lds_lists);
+    append_key(key, ctx->uniform_max_cells);
+    append_key(key, ctx->xcd_remap);
+    return key;
+}
+
+// Captures the step into a graph on the context's private stream with the
+// context's scratch (grown first, outside the capture) and keeps it.
+int capture_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
+                 const brt_steer_params* steer, float feature_rate, float dt, uint32_t step,
+                 const brt_step_buffers* b, std::vector<unsigned char>&& key) {
+    const uint32_t n = b->n;
+    const bool do_features = b->features && b->features_out && feature_rate != 0.0f;
+    const bool two_pass = !(resolve_fused(ctx, prox) && !prox->use_knn);
+    const bool pack = !prox->use_knn && resolve_lds_lists(ctx, prox) != 0u;
+    int rc = ensure_scratch(ctx, n, brt::make_sort_plan(n, grid->num_cells));
+    if (rc) return rc;
+    if (do_features && two_pass && (rc = ensure_tmp(ctx, (size_t)n * 16u))) return rc;
+    if (pack && (rc = ensure_xyz(ctx, n))) return rc;
+    if (!ctx->cap_stream)
+        BRT_HIP(hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (ctx->graphs.size() == 2) {
+        destroy_step_graph(ctx->graphs.front());
+        ctx->graphs.erase(ctx->graphs.begin());
+    }
+    StepScratch sc{ctx->meta, ctx->tmp, ctx->xyz};
+    BRT_HIP(hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    rc = step_core(ctx, grid, prox, steer, feature_rate, dt, step, b, ctx->cap_stream, &sc);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->cap_stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_err(ctx, e, "hipStreamEndCapture");
+    brt_context::StepGraph ng;
+    ng.graph = g;
+    ng.key = std::move(key);
+    ng.tmp = ctx->tmp;
+    ng.meta = ctx->meta;
+    ng.xyz = ctx->xyz;
+    const hipError_t ie = hipGraphInstantiate(&ng.exec, ng.graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        destroy_step_graph(ng);
+        return hip_err(ctx, ie, "hipGraphInstantiate");
+    }
+    ctx->graphs.push_back(std::move(ng));
+    return BRT_OK;
+}
+
+}  // namespace
+
 int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
              const brt_steer_params* steer, float feature_rate, float dt, uint32_t step,
              const brt_step_buffers* b, void* stream) {
-    return step_core(ctx, grid, prox, steer, feature_rate, dt, step, b, stream, nullptr);
+    if (!ctx) return BRT_ERR_INVALID_ARGUMENT;
+    hipStream_t s = (hipStream_t)stream;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    // graphs for steps without phase timing, outside any capture of the caller's
+    if (!ctx->step_graph || ctx->timing || !grid || !prox || !steer || !b || b->n == 0 ||
+        hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return step_core(ctx, grid, prox, steer, feature_rate, dt, step, b, stream, nullptr);
+    std::vector<unsigned char> key = step_key(ctx, grid, prox, steer, feature_rate, dt, step, b);
+    size_t hit = ctx->graphs.size();
+    for (size_t k = 0; k < ctx->graphs.size(); ++k) {
+        const brt_context::StepGraph& g = ctx->graphs[k];
+        if (g.key == key && g.tmp == ctx->tmp && g.meta == ctx->meta && g.xyz == ctx->xyz) hit = k;
+    }
+    if (hit == ctx->graphs.size()) {
+        // the first call with a key runs (and checks its arguments) directly;
+        // the second captures it (a key seen once, e.g. moving seek targets
+        // that change the parameters every step, is never captured)
+        if (key != ctx->last_key) {
+            ctx->last_key = key;
+            return step_core(ctx, grid, prox, steer, feature_rate, dt, step, b, stream, nullptr);
+        }
+        int rc = use_device(ctx);
+        if (rc) return rc;
+        if ((rc = capture_step(ctx, grid, prox, steer, feature_rate, dt, step, b, std::move(key)))) return rc;
+    } else if (hit + 1 != ctx->graphs.size()) {
+        std::swap(ctx->graphs[hit], ctx->graphs.back());
+    }
+    BRT_HIP(hipGraphLaunch(ctx->graphs.back().exec, s), "hipGraphLaunch");
+    return BRT_OK;
 }
 
 int brt_step_noalloc(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,

@@ -33,6 +33,21 @@ struct brt_context {
     uint32_t uniform_max_cells = 1;
     uint32_t xcd_remap = 1;
     int lds_lists = -1;         // -1: automatic (resolve_lds_lists)
+    // brt_step's captured graphs (BRT_OPT_STEP_GRAPH), most recently used last,
+    // and the key of the last call (a graph is captured on the second call
+    // with a key)
+    int step_graph = 0;
+    hipStream_t cap_stream = nullptr;
+    struct StepGraph {
+        std::vector<unsigned char> key;
+        void* tmp = nullptr;  // the scratch the graph was captured with
+        uint32_t* meta = nullptr;
+        float* xyz = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<StepGraph> graphs;
+    std::vector<unsigned char> last_key;
 };
 
 // Automatic launch shapes, from the K = 32 and K = 64 workloads measured on

@@ -81,7 +81,8 @@ __global__ void __launch_bounds__(256) k_features(uint32_t W, float rate, uint32
 // order (sfeat[j] = fin[hash[j].y], gathered with the state), so the walk reads
 // them next to their positions instead of at random agent IDs.
 template <bool APPLY, bool FEAT = false, bool SFEAT = false>
-__global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, brt_steer_params sp, float dt,
+__global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, brt_steer_params sp, SteerCuts cuts,
+                                               float dt,
                                                uint32_t step, uint32_t n, const float4* __restrict__ spos,
                                                const float4* __restrict__ sfwd, const float4* __restrict__ ssa,
                                                const uint2* __restrict__ hash,
@@ -104,6 +105,7 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
     const uint32_t count = w0.x < W - 1u ? w0.x : W - 1u;
     const Weights wts = weights_of(sp);
     const bool any = wts.s != 0.0f || wts.a != 0.0f || wts.c != 0.0f;
+    const float margin = cut_margin(myFwd);
 
     SteerAccum acc;
     acc.init();
@@ -133,7 +135,7 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
                 const uint32_t e = 4u * q + t;
                 if (e >= 1u && e <= count && jj[t] < n) {
                     const V3 offset = xyz(qp[t]) - myPos;
-                    acc.add(sp, wts, myFwd, offset, dot(offset, offset), xyz(qf[t]));
+                    acc.add_cut(sp, cuts, wts, myFwd, margin, offset, dot(offset, offset), xyz(qf[t]));
                 }
             }
             if (FEAT) {
@@ -194,22 +196,55 @@ __global__ void __launch_bounds__(256) k_apply(brt_grid_params g, brt_steer_para
 
 }  // namespace
 
+namespace {
+
+// The smallest float t >= 0 with sqrtf(t) >= m, so that sqrtf(d2) < m <=> d2 < t
+// for every float d2 (NaN included: both false).
+float below_cut(float m) {
+    if (m != m) return -INFINITY;
+    if (m <= 0.0f) return 0.0f;
+    if (m == INFINITY) return INFINITY;
+    float t = m * m;
+    if (t == INFINITY) t = 3.40282347e38f;
+    while (t > 0.0f && sqrtf(nextafterf(t, -INFINITY)) >= m) t = nextafterf(t, -INFINITY);
+    while (sqrtf(t) < m) t = nextafterf(t, INFINITY);
+    return t;
+}
+
+}  // namespace
+
+SteerCuts steer_cuts(const brt_steer_params& sp) {
+    SteerCuts c{};
+    c.tmin = below_cut(sp.sensing_min_radius);
+    const float* b[3] = {sp.separation, sp.alignment, sp.cohesion};
+    for (int k = 0; k < 3; ++k) {
+        c.tmax[k] = brt_radius_threshold(b[k][1]);
+        c.cmax[k] = b[k][2];
+    }
+    c.same = (c.tmax[0] == c.tmax[1] && c.tmax[0] == c.tmax[2] && c.cmax[0] == c.cmax[1] && c.cmax[0] == c.cmax[2])
+                 ? 1u
+                 : 0u;
+    return c;
+}
+
 hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_params& sp, float dt, uint32_t step,
                         uint32_t n, const float4* spos, const float4* sfwd, const float4* ssa, const uint2* hash,
                         const uint32_t* lists, float2* wander, float4* pos_out, float4* fwd_out, float4* sa_out,
                         float4* force_out, uint32_t xcd_remap, hipStream_t s, float feature_rate,
                         const float4* features_in, float4* features_out, const float4* sorted_features) {
     AgentOut o{pos_out, fwd_out, sa_out, force_out, wander};
+    const SteerCuts cuts = steer_cuts(sp);
     if (features_in && sorted_features)
         hipLaunchKernelGGL((k_steer<true, true, true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp,
-                           dt, step, n, spos, sfwd, ssa, hash, lists, o, xcd_remap, feature_rate, features_in,
+                           cuts, dt, step, n, spos, sfwd, ssa, hash, lists, o, xcd_remap, feature_rate, features_in,
                            features_out, sorted_features);
     else if (features_in)
-        hipLaunchKernelGGL((k_steer<true, true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt,
+        hipLaunchKernelGGL((k_steer<true, true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, cuts, dt,
                            step, n, spos, sfwd, ssa, hash, lists, o, xcd_remap, feature_rate, features_in,
                            features_out, nullptr);
     else
-        hipLaunchKernelGGL((k_steer<true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt, step,
+        hipLaunchKernelGGL((k_steer<true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, cuts, dt,
+                           step,
                            n, spos, sfwd, ssa, hash, lists, o, xcd_remap, 0.0f, nullptr, nullptr);
     return hipGetLastError();
 }
@@ -219,7 +254,8 @@ hipError_t launch_steer_forces(uint32_t K, const brt_steer_params& sp, float dt,
                                float2* wander, float4* force_out, uint32_t xcd_remap, hipStream_t s) {
     AgentOut o{nullptr, nullptr, nullptr, force_out, wander};
     brt_grid_params g{};
-    hipLaunchKernelGGL((k_steer<false>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, dt, step, n,
+    hipLaunchKernelGGL((k_steer<false>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp,
+                       steer_cuts(sp), dt, step, n,
                        spos, sfwd, sfwd, hash, lists, o, xcd_remap, 0.0f, nullptr, nullptr);
     return hipGetLastError();
 }

@@ -142,6 +142,7 @@ struct ProxArgs {
     uint32_t* lists;
     // fused steering
     brt_steer_params sp;
+    SteerCuts cuts;
     float dt;
     uint32_t step;
     AgentOut o;
@@ -760,6 +761,7 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
             SteerAccum acc;
             acc.init();
             const Weights wts = sg.w;
+            const float margin = cut_margin(myFwd);
             if (wts.s != 0.0f || wts.a != 0.0f || wts.c != 0.0f) {
                 const __amdgpu_buffer_rsrc_t rpos = float4_rsrc(a.spos, a.n);
                 const __amdgpu_buffer_rsrc_t rfwd = float4_rsrc(a.sfwd, a.n);
@@ -776,7 +778,7 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
                     for (int u = 0; u < SG; ++u) {
                         if (t + u < L.cnt) {
                             const V3 offset = xyz(q[u]) - myPos;
-                            acc.add(a.sp, wts, myFwd, offset, dot(offset, offset), xyz(f[u]));
+                            acc.add_cut(a.sp, a.cuts, wts, myFwd, margin, offset, dot(offset, offset), xyz(f[u]));
                         }
                     }
                 }
@@ -931,6 +933,7 @@ hipError_t launch_neighbors_steer(const brt_grid_params& g, const brt_prox_param
     a.sfwd = sfwd;
     a.ssa = ssa;
     a.sp = sp;
+    a.cuts = steer_cuts(sp);
     a.dt = dt;
     a.step = step;
     a.o = AgentOut{pos_out, fwd_out, sa_out, nullptr, wander};
@@ -954,6 +957,7 @@ hipError_t launch_batch_neighbors_steer(const BatchView& b, uint32_t n, const br
     a.sfwd = sfwd;
     a.ssa = ssa;
     a.sp = sp;
+    a.cuts = steer_cuts(sp);
     a.dt = dt;
     a.step = 0;
     a.o = AgentOut{pos_out, fwd_out, sa_out, nullptr, wander};

@@ -35,6 +35,34 @@ __device__ __forceinline__ Weights weights_of(const brt_steer_params& sp) {
     return Weights{sp.separation[0], sp.alignment[0], sp.cohesion[0]};
 }
 
+// The radius tests of inBoidNeighborhood as exact cuts on the squared
+// distance (steer_cuts, host): sqrtf(d2) < minR <=> d2 < tmin and
+// sqrtf(d2) > maxR_b <=> d2 > tmax[b] for every float d2, because sqrtf is
+// correctly rounded and monotone; cmax[b] = cosMax of behaviour b (separation,
+// alignment, cohesion); same: the three share maxR and cosMax.
+struct SteerCuts {
+    float tmin;
+    float tmax[3];
+    float cmax[3];
+    uint32_t same;
+};
+
+// inBoidNeighborhood of one behaviour, decided without a square root when the
+// answer is certain: the radius tests by the cuts, the angle test
+// cosAngle > cosMax by ca = dot(fwd, off) * rsq(d2).  The forward's cosAngle =
+// dot(fwd, off * (1/sqrtf(d2))) and ca both lie within about 10 roundings
+// (2^-24 relative) of dot(fwd, off)/sqrt(d2), i.e. within 2e-6 |fwd| of each
+// other; the caller passes margin = 1e-5 |fwd|_1 and fast = d2 normal-range and
+// ca finite.  1: in, 0: out, 2: undecided (evaluate the forward's expression).
+// (selects, no branches)
+__device__ __forceinline__ uint32_t cut_test(float d2, float tmin, float tmax, float ca, float cmax, float margin,
+                                             bool fast) {
+    const float gap = ca - cmax;
+    const bool sure = fast & (fabsf(gap) > margin);
+    const uint32_t ang = sure ? (gap > 0.0f ? 1u : 0u) : 2u;
+    return d2 < tmin ? 1u : (d2 > tmax ? 0u : ang);
+}
+
 struct SteerAccum {
     V3 sSep, sAli, sCoh;
     uint32_t nSep, nAli, nCoh;
@@ -66,7 +94,46 @@ struct SteerAccum {
             ++nCoh;
         }
     }
+
+    // add() with the neighbourhood tests decided by the cuts (same decisions,
+    // same sums); margin = 1e-5 (|fwd.x| + |fwd.y| + |fwd.z|) of the agent.
+    __device__ __forceinline__ void add_cut(const brt_steer_params& sp, const SteerCuts& c, Weights w, V3 myFwd,
+                                            float margin, V3 offset, float d2, V3 fwdj) {
+        const float ca = dot(myFwd, offset) * __builtin_amdgcn_rsqf(d2);
+        const bool fast = (d2 >= 0x1p-100f) & (d2 <= 0x1p+100f) & __builtin_isfinite(ca);
+        uint32_t ts = cut_test(d2, c.tmin, c.tmax[0], ca, c.cmax[0], margin, fast), ta = ts, tc = ts;
+        if (!c.same) {
+            ta = cut_test(d2, c.tmin, c.tmax[1], ca, c.cmax[1], margin, fast);
+            tc = cut_test(d2, c.tmin, c.tmax[2], ca, c.cmax[2], margin, fast);
+        }
+        if ((ts | ta | tc) & 2u) {
+            const float dist = sqrtf(d2);
+            const V3 nrm = offset * (1.0f / dist);
+            const float cosAngle = dot(myFwd, nrm);
+            const float minR = sp.sensing_min_radius;
+            if (ts == 2u) ts = inBoid(dist, cosAngle, minR, sp.separation[1], sp.separation[2]) ? 1u : 0u;
+            if (ta == 2u) ta = inBoid(dist, cosAngle, minR, sp.alignment[1], sp.alignment[2]) ? 1u : 0u;
+            if (tc == 2u) tc = inBoid(dist, cosAngle, minR, sp.cohesion[1], sp.cohesion[2]) ? 1u : 0u;
+        }
+        if (w.s != 0.0f && ts) {
+            sSep = sSep + offset / (-d2);
+            ++nSep;
+        }
+        if (w.a != 0.0f && ta) {
+            sAli = sAli + fwdj;
+            ++nAli;
+        }
+        if (w.c != 0.0f && tc) {
+            sCoh = sCoh + offset;
+            ++nCoh;
+        }
+    }
 };
+
+__device__ __forceinline__ float cut_margin(V3 fwd) { return 1e-5f * ((fabsf(fwd.x) + fabsf(fwd.y)) + fabsf(fwd.z)); }
+
+// host: the cuts of the steer params
+SteerCuts steer_cuts(const brt_steer_params& sp);
 
 struct AgentOut {
     float4* pos_out;

@@ -31,6 +31,7 @@ OPT_UNIFORM_MAX_CELLS = 2
 OPT_XCD_REMAP = 3
 OPT_LDS_LISTS = 4
 OPT_DEVICE_FLAGS = 5
+OPT_STEP_GRAPH = 6
 DEVFLAG_SORT_LOOKBACK = 1
 PHASE_NAMES = ("hash_sort", "reorder_ranges", "neighbors", "steer_integrate", "features")
 

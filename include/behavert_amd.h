@@ -137,6 +137,14 @@ const char* brt_version(void);
  * see brt_context_check); get synchronises the device and reads it, set writes
  * it (for tests of the error path). */
 #define BRT_OPT_DEVICE_FLAGS 5
+/* BRT_OPT_STEP_GRAPH: brt_step captures its kernels into a hipGraph on the
+ * second call with the same buffers, parameters and options (the step number
+ * counts only with wander on) and replays the graph from then on, one launch
+ * per step (two graphs are kept, for a feature update's swapped buffers;
+ * steps with phase timing on launch their kernels directly).  Default 0: on
+ * gfx950 with ROCm 7 a replay measured 5-7 us slower per step than the direct
+ * launches (1,048,576 and 65,536 boids). */
+#define BRT_OPT_STEP_GRAPH 6
 int brt_context_set_option(brt_context* ctx, int option, int value);
 int brt_context_get_option(const brt_context* ctx, int option, int* value);
 /* The launch shape brt_step and brt_prox_neighbors use for these proximity

@@ -5,6 +5,7 @@ FP64 evaluation (per agent, norm-wise, floored at each quantity's scale:
 force 1.0, smooth_accel 0.4 = rate * unit force, forward 1.0, pos 1.0)."""
 import numpy as np
 import pytest
+import torch
 
 from behavert_amd import workloads as W
 from behavert_amd.engine import Flock
@@ -278,3 +279,75 @@ def test_automatic_step_shape(ctx):
     finally:
         ctx.set_option(OPT_FUSED_STEP, old[0])
         ctx.set_option(OPT_LDS_LISTS, old[1])
+
+
+@pytest.mark.parametrize("make", [W.plain_4096, W.blobs_1m, "features"])
+def test_step_graph_replays_equal_direct_launches(ctx, make):
+    """brt_step replayed from its captured graph (from the second step on; with
+    the feature update, two graphs alternate over the swapped feature buffers)
+    equals the direct launches bit for bit at every step, with eager work and
+    synchronisations between the replays."""
+    from behavert_amd.types import OPT_STEP_GRAPH
+    if make == "features":
+        w = W.full_stack_4m(n=1 << 16)
+        w.side = 640.0 * (1 / 64) ** (1 / 3)
+        w.state = W.uniform_flock(w.n, w.side, 0)
+        w.grid = W.cube_grid(w.side)
+        w.steer.wander_weight = 0.0
+        w.moving_targets = False
+        w.wander = None
+    else:
+        w = make()
+    runs = []
+    old = ctx.get_option(OPT_STEP_GRAPH)
+    try:
+        for g in (0, 1):
+            ctx.set_option(OPT_STEP_GRAPH, g)
+            fl = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander)
+            scratch = torch.zeros(1 << 20, device="cuda")
+            hist = []
+            for t in range(6):
+                fl.step(w.dt, t, w.feature_rate)
+                scratch.add_(1.0)            # eager work between the replays
+                torch.cuda.synchronize()
+                st = fl.state()
+                hist.append({k: v.copy() for k, v in st.items() if v is not None})
+            runs.append(hist)
+    finally:
+        ctx.set_option(OPT_STEP_GRAPH, old)
+    for t, (a, b) in enumerate(zip(*runs)):
+        for k in a:
+            assert first_mismatch(b[k].view(np.uint32), a[k].view(np.uint32)) is None, (t, k)
+
+
+@pytest.mark.parametrize("cos_max,min_r,radius", [(-1.0, 0.0, 4.0), (0.0, 1.0, 2.0), (0.5, 1.0, 3.0)])
+def test_neighbourhood_cuts_on_exact_boundaries(ctx, orc, cos_max, min_r, radius):
+    """A lattice flock with axis-aligned headings: many neighbours sit exactly at
+    the sensing min radius and at the behaviour radius, and exactly
+    (anti)parallel to the heading (cosAngle exactly -1, 0 or 1); the steering
+    passes decide the neighbourhood tests from squared-distance cuts and an
+    approximate cosine, and must match the oracle's sqrt/cosine tests bit for
+    bit (fused step and two-pass step)."""
+    from behavert_amd.types import OPT_FUSED_STEP
+    g = np.arange(20, dtype=np.float32) + np.float32(0.5)
+    xyz = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    n = len(xyz)
+    rng = np.random.default_rng(17)
+    axes = np.eye(3, dtype=np.float32)[rng.integers(0, 3, n)] * rng.choice([-1.0, 1.0], n)[:, None].astype(np.float32)
+    speeds = rng.uniform(0.5, 1.0, n).astype(np.float32)
+    st = W.make_state(xyz, axes, speeds)
+    grid = W.cube_grid(20.0)
+    prox = ProxParams.make(32, 4.0)
+    sp = SteerParams.make(weights=(1.0, 0.8, 0.6), radius=radius, cos_max=cos_max, min_radius=min_r)
+    w = W.Workload("lattice", n, 20.0, grid, prox, sp, 0.05, 1, st)
+    old = ctx.get_option(OPT_FUSED_STEP)
+    try:
+        for fused in (1, 0):
+            ctx.set_option(OPT_FUSED_STEP, fused)
+            fl = run_gpu_steps(ctx, w, 2)
+            hf, _ = run_oracle_steps(orc, w, 2)
+            got, ref = fl.state(), hf.state()
+            for k in ("pos", "forward", "smooth_accel"):
+                assert first_mismatch(got[k].view(np.uint32), ref[k].view(np.uint32)) is None, (fused, k)
+    finally:
+        ctx.set_option(OPT_FUSED_STEP, old)
