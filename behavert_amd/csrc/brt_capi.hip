@@ -128,9 +128,10 @@ int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan) {
 // hash (+ histograms) then the onesweep passes; the sorted pairs end in `hash`.
 // meta: sort metadata (sort_meta_words), tmp: ping-pong uint2[n]; both given
 // (meta == nullptr: the context's, allocated as needed).
+// ev_start / ev_stop (phase timing): bound to the first and the last kernel.
 int run_hash_sort(brt_context* ctx, const brt_grid_params* g, const float* pos, uint32_t n, uint32_t* hash,
                   uint32_t* cell_start, uint32_t* cell_end, hipStream_t s, uint32_t* meta = nullptr,
-                  void* tmp = nullptr) {
+                  void* tmp = nullptr, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr) {
     brt::SortPlan plan = brt::make_sort_plan(n, g->num_cells);
     if (!meta) {
         int rc = ensure_scratch(ctx, n, plan);
@@ -138,16 +139,19 @@ int run_hash_sort(brt_context* ctx, const brt_grid_params* g, const float* pos, 
         meta = ctx->meta;
         tmp = ctx->tmp;
     }
+    brt::launch_mark().start = ev_start;
     BRT_HIP(brt::launch_prepare(meta, brt::sort_meta_words(plan), cell_start, cell_end, g->num_cells, s),
             "k_prepare");
     uint2* A = reinterpret_cast<uint2*>(hash);
     uint2* B = reinterpret_cast<uint2*>(tmp);
     uint2* first = (plan.passes & 1u) ? B : A;
+    if (plan.passes == 0u) brt::launch_mark().stop = ev_stop;
     BRT_HIP(brt::launch_hash_hist(*g, reinterpret_cast<const float4*>(pos), n, first, meta, plan, s),
             "k_hash_hist");
     uint2* src = first;
     uint2* dst = first == A ? B : A;
     for (uint32_t q = 0; q < plan.passes; ++q) {
+        if (q + 1u == plan.passes) brt::launch_mark().stop = ev_stop;
         BRT_HIP(brt::launch_onesweep(src, dst, n, q, plan, meta, ctx->dev_flags, s), "k_onesweep");
         std::swap(src, dst);
     }
@@ -171,10 +175,11 @@ int check_dev_flags(brt_context* ctx) {
 void fold_events(brt_context* ctx) {
     for (size_t k = 0; k < ctx->events_used; ++k) {
         auto& ev = ctx->events[k];
-        (void)hipEventSynchronize(ev[BRT_NUM_PHASES]);
         for (int p = 0; p < BRT_NUM_PHASES; ++p) {
+            if (!(ctx->events_mask[k] >> p & 1u)) continue;
             float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, ev[p], ev[p + 1]) == hipSuccess) ctx->phase_ms[p] += ms;
+            (void)hipEventSynchronize(ev[2 * p + 1]);
+            if (hipEventElapsedTime(&ms, ev[2 * p], ev[2 * p + 1]) == hipSuccess) ctx->phase_ms[p] += ms;
         }
         ctx->phase_steps += 1;
     }
@@ -899,17 +904,32 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
     if ((rc = use_device(ctx))) return rc;
     hipStream_t s = (hipStream_t)stream;
 
-    std::array<hipEvent_t, BRT_NUM_PHASES + 1>* ev = nullptr;
+    // phase timing: each phase's start and stop events are bound to its first
+    // and last kernel (no marker packets between the phases)
+    std::array<hipEvent_t, 2 * BRT_NUM_PHASES>* ev = nullptr;
+    uint32_t* ev_mask = nullptr;
+    struct MarkGuard {
+        ~MarkGuard() { brt::launch_mark() = brt::LaunchMark{}; }
+    } mark_guard;
+    brt::launch_mark() = brt::LaunchMark{};
     if (ctx->timing && !given) {
         if (ctx->events_used == 4096) fold_events(ctx);
         if (ctx->events_used == ctx->events.size()) {
-            std::array<hipEvent_t, BRT_NUM_PHASES + 1> a;
+            std::array<hipEvent_t, 2 * BRT_NUM_PHASES> a;
             for (auto& e : a) BRT_HIP(hipEventCreate(&e), "hipEventCreate");
             ctx->events.push_back(a);
+            ctx->events_mask.push_back(0u);
         }
+        ev_mask = &ctx->events_mask[ctx->events_used];
         ev = &ctx->events[ctx->events_used++];
-        BRT_HIP(hipEventRecord((*ev)[0], s), "hipEventRecord");
+        *ev_mask = 0u;
     }
+    // arms the phase's events for the next launch (the phase's only kernel)
+    auto mark_phase = [&](int phase) {
+        if (!ev) return;
+        brt::launch_mark() = brt::LaunchMark{(*ev)[2 * phase], (*ev)[2 * phase + 1]};
+        *ev_mask |= 1u << phase;
+    };
     // the two-pass path gathers the features into sorted order with the state
     // (into the sort's ping-pong buffer, free once the sort is done), so the
     // steering pass reads the neighbours' features by sorted index
@@ -930,10 +950,12 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
             return rc;
     } else {
         // phase 0: prepare (scratch + empty-cell fill) + hash + sort
-        if ((rc = run_hash_sort(ctx, grid, b->pos, n, b->hash, b->cell_start, b->cell_end, s))) return rc;
+        if ((rc = run_hash_sort(ctx, grid, b->pos, n, b->hash, b->cell_start, b->cell_end, s, nullptr, nullptr,
+                                ev ? (*ev)[0] : nullptr, ev ? (*ev)[1] : nullptr)))
+            return rc;
+        if (ev) *ev_mask |= 1u;
         tmp = ctx->tmp;
     }
-    if (ev) BRT_HIP(hipEventRecord((*ev)[1], s), "hipEventRecord");
     // phase 1: reorder pos/forward/smooth_accel + cell ranges
     brt::ReorderArgs f{};
     f.count = 3;
@@ -949,13 +971,14 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
         f.in[3] = reinterpret_cast<const float4*>(b->features);
         f.out[3] = reinterpret_cast<float4*>(tmp);
     }
+    mark_phase(1);
     BRT_HIP(brt::launch_reorder_ranges(reinterpret_cast<const uint2*>(b->hash), n, f, grid->num_cells, b->cell_start,
                                        b->cell_end, s),
             "k_reorder_ranges");
-    if (ev) BRT_HIP(hipEventRecord((*ev)[2], s), "hipEventRecord");
     const float T = brt_radius_threshold(prox->search_radius);
     if (!two_pass) {
         // phase 2: neighbour lists + steering + integration in one pass
+        mark_phase(2);
         BRT_HIP(brt::launch_neighbors_steer(*grid, *prox, T, *steer, dt, step,
                                             reinterpret_cast<const float4*>(b->sorted_pos),
                                             reinterpret_cast<const float4*>(b->sorted_forward),
@@ -966,17 +989,16 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
                                             reinterpret_cast<float4*>(b->smooth_accel), ctx->uniform_max_cells,
                                             ctx->xcd_remap, lds, s, xyz),
                 "k_prox<fused>");
-        if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
-        if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
     } else {
         // phase 2: neighbour lists
+        mark_phase(2);
         BRT_HIP(brt::launch_neighbors(*grid, *prox, T, reinterpret_cast<const float4*>(b->sorted_pos),
                                       reinterpret_cast<const uint2*>(b->hash), b->cell_start, b->cell_end, n,
                                       b->lists, ctx->uniform_max_cells, ctx->xcd_remap, lds, s, xyz),
                 "k_neighbors");
-        if (ev) BRT_HIP(hipEventRecord((*ev)[3], s), "hipEventRecord");
         // phase 3: fused steering + integration, scattered back to agent-ID order,
         // with the feature update in the same pass over the lists
+        mark_phase(3);
         BRT_HIP(brt::launch_steer(*grid, prox->max_neighbors, *steer, dt, step, n,
                                   reinterpret_cast<const float4*>(b->sorted_pos),
                                   reinterpret_cast<const float4*>(b->sorted_forward),
@@ -989,16 +1011,16 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
                                   do_features ? reinterpret_cast<float4*>(b->features_out) : nullptr,
                                   sorted_features ? reinterpret_cast<const float4*>(tmp) : nullptr),
                 "k_steer");
-        if (ev) BRT_HIP(hipEventRecord((*ev)[4], s), "hipEventRecord");
         do_features = false;
     }
     // phase 4: BehaviorClustering feature update (fused path)
-    if (do_features)
+    if (do_features) {
+        mark_phase(4);
         BRT_HIP(brt::launch_features(prox->max_neighbors, feature_rate, n, reinterpret_cast<const uint2*>(b->hash),
                                      b->lists, reinterpret_cast<const float4*>(b->features),
                                      reinterpret_cast<float4*>(b->features_out), s),
                 "k_features");
-    if (ev) BRT_HIP(hipEventRecord((*ev)[5], s), "hipEventRecord");
+    }
     return BRT_OK;
 }
 

@@ -353,19 +353,23 @@ size_t sort_meta_words(const SortPlan& p) {
     return (size_t)kMetaStatus + (size_t)p.passes * p.tiles * kRadix;
 }
 
+LaunchMark& launch_mark() {
+    static thread_local LaunchMark m;
+    return m;
+}
+
 hipError_t launch_prepare(uint32_t* meta, size_t meta_words, uint32_t* cell_start, uint32_t* cell_end,
                           uint32_t num_cells, hipStream_t s) {
     size_t work = meta_words > (size_t)num_cells / 4 ? meta_words : (size_t)num_cells / 4;
     uint32_t grid = grid_for((uint32_t)(work > 0xFFFFFFFFull ? 0xFFFFFFFFu : work), 256, 2048);
-    hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(256), 0, s, meta, (uint32_t)meta_words, cell_start, cell_end,
-                       num_cells);
+    BRT_LAUNCH(k_prepare, dim3(grid), dim3(256), 0, s, meta, (uint32_t)meta_words, cell_start, cell_end, num_cells);
     return hipGetLastError();
 }
 
 hipError_t launch_hash_hist(const brt_grid_params& g, const float4* pos, uint32_t n, uint2* out, uint32_t* meta,
                             const SortPlan& plan, hipStream_t s) {
     uint32_t grid = grid_for(n, 256, hist_blocks(n));
-    hipLaunchKernelGGL(k_hash_hist, dim3(grid), dim3(256), 0, s, g, pos, n, out, meta + kMetaHist, plan);
+    BRT_LAUNCH(k_hash_hist, dim3(grid), dim3(256), 0, s, g, pos, n, out, meta + kMetaHist, plan);
     return hipGetLastError();
 }
 
@@ -392,20 +396,18 @@ hipError_t launch_onesweep(const uint2* in, uint2* out, uint32_t n, uint32_t pas
                            uint32_t* meta, uint32_t* dev_flags, hipStream_t s) {
     uint32_t* status = meta + kMetaStatus + (size_t)pass * plan.tiles * kRadix;
     if (plan.items == 8u)
-        hipLaunchKernelGGL(k_onesweep<8>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n, plan.shift[pass],
-                           plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass,
-                           dev_flags);
+        BRT_LAUNCH(k_onesweep<8>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n, plan.shift[pass],
+                   plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass, dev_flags);
     else
-        hipLaunchKernelGGL(k_onesweep<16>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n, plan.shift[pass],
-                           plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass,
-                           dev_flags);
+        BRT_LAUNCH(k_onesweep<16>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n, plan.shift[pass],
+                   plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass, dev_flags);
     return hipGetLastError();
 }
 
 hipError_t launch_reorder_ranges(const uint2* hash, uint32_t n, const ReorderArgs& f, uint32_t num_cells,
                                  uint32_t* cell_start, uint32_t* cell_end, hipStream_t s) {
     uint32_t grid = (n + 255) / 256;
-    hipLaunchKernelGGL(k_reorder_ranges, dim3(grid), dim3(256), 0, s, hash, n, f, num_cells, cell_start, cell_end);
+    BRT_LAUNCH(k_reorder_ranges, dim3(grid), dim3(256), 0, s, hash, n, f, num_cells, cell_start, cell_end);
     return hipGetLastError();
 }
 

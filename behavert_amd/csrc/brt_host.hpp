@@ -25,7 +25,10 @@ struct brt_context {
     uint32_t* dev_flags = nullptr;  // sticky device-side failure flags (BRT_DEVFLAG_*)
     void* scratch = nullptr;        // this device's no-allocation step scratch (a static device array)
     bool timing = false;
-    std::vector<std::array<hipEvent_t, BRT_NUM_PHASES + 1>> events;
+    // per timed step: a start and a stop event per phase (bound to the
+    // phase's first and last kernel) and the mask of the phases that ran
+    std::vector<std::array<hipEvent_t, 2 * BRT_NUM_PHASES>> events;
+    std::vector<uint32_t> events_mask;
     size_t events_used = 0;
     double phase_ms[BRT_NUM_PHASES] = {0, 0, 0, 0, 0};
     uint32_t phase_steps = 0;

@@ -1,6 +1,7 @@
 // brt_internal.hpp -- host-side launchers shared between the kernel TUs and the C ABI.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -8,6 +9,29 @@
 #include "../../include/behavert_amd.h"
 
 namespace brt {
+
+// Phase timing without marker packets: the caller arms a start and/or stop
+// event for the next kernel a launcher enqueues on this thread, and that
+// launch binds them to the kernel's own dispatch (hipExtLaunchKernelGGL), so
+// the events carry the kernel's start and end timestamps.  A launch consumes
+// the mark; launches with no mark armed are plain launches.
+struct LaunchMark {
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+LaunchMark& launch_mark();
+
+#define BRT_LAUNCH(kernel, grid, block, shm, stream, ...)                                                \
+    do {                                                                                                \
+        ::brt::LaunchMark& mark_ = ::brt::launch_mark();                                                \
+        if (mark_.start || mark_.stop) {                                                                \
+            const ::brt::LaunchMark armed_ = mark_;                                                     \
+            mark_ = ::brt::LaunchMark{};                                                                \
+            hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, armed_.start, armed_.stop, 0u, __VA_ARGS__); \
+        } else {                                                                                        \
+            hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__);                          \
+        }                                                                                               \
+    } while (0)
 
 constexpr int kSortThreads = 256;            // 4 wave64s per tile
 constexpr int kSortItems = 16;               // keys per thread

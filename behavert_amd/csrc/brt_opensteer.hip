@@ -235,17 +235,17 @@ hipError_t launch_steer(const brt_grid_params& g, uint32_t K, const brt_steer_pa
     AgentOut o{pos_out, fwd_out, sa_out, force_out, wander};
     const SteerCuts cuts = steer_cuts(sp);
     if (features_in && sorted_features)
-        hipLaunchKernelGGL((k_steer<true, true, true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp,
+        BRT_LAUNCH((k_steer<true, true, true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp,
                            cuts, dt, step, n, spos, sfwd, ssa, hash, lists, o, xcd_remap, feature_rate, features_in,
                            features_out, sorted_features);
     else if (features_in)
-        hipLaunchKernelGGL((k_steer<true, true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, cuts, dt,
+        BRT_LAUNCH((k_steer<true, true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, cuts, dt,
                            step, n, spos, sfwd, ssa, hash, lists, o, xcd_remap, feature_rate, features_in,
                            features_out, nullptr);
     else
-        hipLaunchKernelGGL((k_steer<true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, cuts, dt,
+        BRT_LAUNCH((k_steer<true>), dim3((n + 255) / 256), dim3(256), 0, s, g, list_stride(K), sp, cuts, dt,
                            step,
-                           n, spos, sfwd, ssa, hash, lists, o, xcd_remap, 0.0f, nullptr, nullptr);
+                           n, spos, sfwd, ssa, hash, lists, o, xcd_remap, 0.0f, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 
@@ -271,7 +271,7 @@ hipError_t launch_apply(const brt_grid_params& g, const brt_steer_params& sp, fl
 
 hipError_t launch_features(uint32_t K, float rate, uint32_t n, const uint2* hash, const uint32_t* lists,
                            const float4* fin, float4* fout, hipStream_t s) {
-    hipLaunchKernelGGL(k_features, dim3((n + 255) / 256), dim3(256), 0, s, list_stride(K), rate, n, hash, lists, fin,
+    BRT_LAUNCH(k_features, dim3((n + 255) / 256), dim3(256), 0, s, list_stride(K), rate, n, hash, lists, fin,
                        fout);
     return hipGetLastError();
 }

@@ -907,7 +907,7 @@ hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, 
         uint32_t B = 64;
         while (B > 8 && (size_t)B * (W + K) * 4u > 60u * 1024u) B /= 2;
         const size_t lds = (size_t)B * (W + K) * 4u;
-        hipLaunchKernelGGL(k_neighbors_knn, dim3((n + B - 1) / B), dim3(B), lds, s, g, K, W, threshold, spos,
+        BRT_LAUNCH(k_neighbors_knn, dim3((n + B - 1) / B), dim3(B), lds, s, g, K, W, threshold, spos,
                            cell_start, cell_end, n, lists);
         return hipGetLastError();
     }
@@ -915,9 +915,9 @@ hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, 
     a.pxyz = pxyz;
     const uint32_t B = prox_block(a.Ws, n);
     if (lds_lists)
-        hipLaunchKernelGGL((k_prox<false, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.Ws * 4u, s, a);
+        BRT_LAUNCH((k_prox<false, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.Ws * 4u, s, a);
     else
-        hipLaunchKernelGGL((k_prox<false, false, false>), dim3((n + 255) / 256), dim3(256), 0, s, a);
+        BRT_LAUNCH((k_prox<false, false, false>), dim3((n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -939,9 +939,9 @@ hipError_t launch_neighbors_steer(const brt_grid_params& g, const brt_prox_param
     a.o = AgentOut{pos_out, fwd_out, sa_out, nullptr, wander};
     const uint32_t B = prox_block(a.Ws, n);
     if (lds_lists)
-        hipLaunchKernelGGL((k_prox<true, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.Ws * 4u, s, a);
+        BRT_LAUNCH((k_prox<true, false, true>), dim3((n + B - 1) / B), dim3(B), (size_t)B * a.Ws * 4u, s, a);
     else
-        hipLaunchKernelGGL((k_prox<true, false, false>), dim3((n + 255) / 256), dim3(256), 0, s, a);
+        BRT_LAUNCH((k_prox<true, false, false>), dim3((n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

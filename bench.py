@@ -67,23 +67,30 @@ PHASE_KERNELS = {"neighbors": "k_prox<false,false,*>", "steer_integrate": "k_ste
                  "features": "k_features", "neighbors_steer": "k_prox<true,false,*>"}
 
 
-PROFILE_NAMES = {"neighbors_steer": "k_prox_fused", "neighbors": "k_prox_lists", "steer_integrate": "k_steer",
-                 "reorder_ranges": "k_reorder_ranges"}
-
-
 def committed_traffic(phase: str, workload: str):
-    """HBM bytes per launch of the phase's kernel from the committed rocprofv3
-    counter summary (profiles/<kernel>_<workload>.md: FETCH_SIZE and WRITE_SIZE
-    scaled by the copy-kernel calibration), or None when there is none."""
-    name = PROFILE_NAMES.get(phase)
-    path = os.path.join(ROOT, "profiles", f"{name}_{workload}.md") if name else ""
-    if not name or not os.path.exists(path):
-        return None
-    with open(path) as f:
-        for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1].split()[0])
+    """HBM bytes per launch of the phase's kernels on the workload from the
+    committed rocprofv3 counter summaries (profiles/*_<workload>.md, a line
+    `traffic_bytes_per_launch[<phase>]: N` = FETCH_SIZE and WRITE_SIZE scaled
+    by the copy-kernel calibration), or None when there is none."""
+    import glob
+
+    key = f"traffic_bytes_per_launch[{phase}]:"
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"*_{workload}.md"))):
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key):
+                    return float(line.split(":", 1)[1].split()[0])
     return None
+
+
+def with_traffic(entry: dict, phase: str, workload: str, n: int) -> dict:
+    """Adds the committed counter traffic of the phase and its ratio to the
+    algorithmic bytes."""
+    t = committed_traffic(phase, workload)
+    entry["traffic"] = t
+    if t is not None and entry.get("bytes_per_agent"):
+        entry["traffic_ratio"] = round(t / (entry["bytes_per_agent"] * n), 3)
+    return entry
 
 
 def phase_table(ctx, w, abytes, fused):
@@ -105,8 +112,9 @@ def phase_table(ctx, w, abytes, fused):
         bkey = "reorder_ranges_features" if (merged and k == "reorder_ranges") else name
         avg = v / max(nsteps, 1)
         gbs = (abytes[bkey] * w.n / 1e9) / (avg / 1e3) if avg > 0 and abytes[bkey] > 0 else 0.0
-        out[name] = {"ms": round(avg, 5), "bytes_per_agent": round(abytes[bkey], 2), "GB/s": round(gbs, 1),
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": PHASE_KERNELS[name]}
+        out[name] = with_traffic({"ms": round(avg, 5), "bytes_per_agent": round(abytes[bkey], 2),
+                                  "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                  "kernel": PHASE_KERNELS[name]}, name, w.name, w.n)
     return out
 
 
@@ -201,11 +209,25 @@ def adjoint_ratio(steps=None, every=None) -> dict:
     adj_ms = e3.elapsed_time(e4) / reps
     step_ms = fwd_ms / T
     ctx.close()
+    # the adjoint step's compulsory bytes per agent (K 32, W 36): k_adjoint_own
+    # reads the hash 8, the sorted state 48, the list 4W and the cotangents by
+    # ID 48, and writes its own terms 48 and neighbour-term vectors 48;
+    # k_adjoint_gather reads the hash 8, the own terms 48 and the list 4W, and
+    # writes the gradients by ID 48 (neighbours' data is cache-resident)
+    from behavert_amd.types import list_stride
+
+    Wl = list_stride(w.prox.max_neighbors)
+    adj_bpa = (200 + 4 * Wl) + (104 + 4 * Wl)
+    adj_gbs = adj_bpa * w.n / 1e9 / (adj_ms / 1e3)
     return {"workload": "differentiable_65k", "n": w.n, "steps": T, "checkpoint_every": K,
             "forward_step_ms": round(step_ms, 5), "adjoint_step_ms": round(adj_ms, 5),
             "step_ratio": round((step_ms + adj_ms) / step_ms, 3),
             "forward_rollout_ms": round(fwd_ms, 3), "backward_rollout_ms": round(bwd_ms, 3),
-            "rollout_ratio": round((fwd_ms + bwd_ms) / fwd_ms, 3)}
+            "rollout_ratio": round((fwd_ms + bwd_ms) / fwd_ms, 3),
+            "roofline": {"bound": "hbm", "scope": "adjoint step (k_adjoint_own + k_adjoint_gather)",
+                         "achieved": round(adj_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(adj_gbs / HBM_PEAK_GBS, 4), "bytes_per_agent": adj_bpa,
+                         "traffic": committed_traffic("adjoint", "differentiable_65k")}}
 
 
 def full_stack(steps=20, warmup=3) -> dict:
@@ -237,8 +259,13 @@ def full_stack(steps=20, warmup=3) -> dict:
     ph = phase_table(ctx, w, abytes, fused=fused)
     ctx.phase_timing(False)
     ctx.close()
+    dom = max((k for k in ph if isinstance(ph[k], dict) and ph[k]["bytes_per_agent"] > 0), key=lambda k: ph[k]["ms"])
+    roofline = {"bound": "hbm", "kernel": ph[dom]["kernel"], "phase": dom, "achieved": ph[dom]["GB/s"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ph[dom]["frac"],
+                "traffic": committed_traffic(dom, w.name), "bytes_per_agent": ph[dom]["bytes_per_agent"],
+                "agents_per_launch": w.n, "avg_launch_ms": ph[dom]["ms"]}
     return {**w.describe(), "ms_per_step": round(1e3 * el / steps, 4), "agent_updates_per_s": w.n * steps / el,
-            "phases": ph, "timed_steps": steps, "step_shape": {"fused": fused, "lds_lists": lds}}
+            "phases": ph, "roofline": roofline, "timed_steps": steps, "step_shape": {"fused": fused, "lds_lists": lds}}
 
 
 def clustering(reps=20, features=8, lp_iterations=10) -> dict:
@@ -415,10 +442,36 @@ def batched(steps=20, loop_steps=3) -> dict:
     loop_ms = timed(loop, loop_steps)
     b.close()
     ctx.close()
+    # the batch's compulsory bytes per agent (the plain step's, with the
+    # batch's own cells per agent and sort passes)
+    from behavert_amd.types import list_stride
+
+    cells = sum(int(g.num_cells) for g in w.grids)
+    D = (max(1, int(cells - 1).bit_length()) + 7) // 8
+    Wl, cpa = list_stride(w.prox.max_neighbors), cells / w.n
+    packed = 12 if cfg.lds_lists else 0
+    bpa = 24 + (8 + 16 * D) + 8 * cpa + (8 + 32 * 3 + packed) + (16 + 4 * Wl + 8 * cpa + 36 + 48)
+    gbs = bpa * w.n / 1e9 / (batch_ms / 1e3)
     return {**w.describe(), "tuned": cfg.as_dict(), "tune_s": round(tune_s, 3),
+            "roofline": {"bound": "hbm", "scope": "batched step", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_agent": round(bpa, 2),
+                         "traffic": committed_traffic("batched", "batched_256")},
             "batched_ms_per_step": round(batch_ms, 4), "batched_agent_updates_per_s": w.n / (batch_ms / 1e3),
             "looped_ms_per_step": round(loop_ms, 4), "looped_agent_updates_per_s": w.n / (loop_ms / 1e3),
             "speedup_per_agent": round(loop_ms / batch_ms, 3), "timed_steps": steps, "looped_timed_steps": loop_steps}
+
+
+def copy_calibration():
+    """Three 256 MiB device-to-device copies: the kernel of known traffic the
+    FETCH_SIZE / WRITE_SIZE counters are scaled by."""
+    import torch
+
+    x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
 
 
 def main():
@@ -445,6 +498,8 @@ def main():
                                       "full_stack": full_stack, "clustering": clustering,
                                       "consumers": consumers}[args.only]()}),
               flush=True)
+        if args.copy_calibration:
+            copy_calibration()
         return
 
     import torch
@@ -501,6 +556,9 @@ def main():
         ctx.set_option(OPT_FUSED_STEP, fused_opt)
         fz = phases["fused_step"]
         fz["sum_ms"] = round(sum(v["ms"] for v in fz.values() if isinstance(v, dict)), 5)
+        # the phases are kernel spans (events bound to each phase's first and
+        # last kernel); their sum against the event-free timed steps
+        fz["sum_over_ms_per_step"] = round(fz["sum_ms"] / ms_per_step, 4)
         dom = max((k for k in fz if isinstance(fz[k], dict) and abytes.get(k, 0) > 0), key=lambda k: fz[k]["ms"])
         roofline = {"bound": "hbm", "kernel": fz[dom]["kernel"], "phase": dom, "achieved": fz[dom]["GB/s"],
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fz[dom]["frac"],
@@ -520,12 +578,7 @@ def main():
     ctx.close()
     del fl
     if args.copy_calibration:
-        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        del x, y
+        copy_calibration()
     extras = {}
     if not args.no_extras:
         extras["adjoint"] = adjoint_ratio()

@@ -510,11 +510,15 @@ int brt_snapshot_load(brt_context* ctx, const char* path, uint32_t flags,
                       const brt_prox_params* prox, const brt_step_buffers* buf,
                       brt_snapshot_info* info, void* stream);
 
-/* Per-phase timing of brt_step with hipEvents recorded on the step's stream
- * (the reference harness's per-phase timers, BehaveRT_Flocking.cpp:171-204).
+/* Per-phase timing of brt_step (the reference harness's per-phase timers,
+ * BehaveRT_Flocking.cpp:171-204): each phase's start and stop hipEvents are
+ * bound to the phase's first and last kernel launch (hipExtLaunchKernelGGL),
+ * so a phase's time is the span of its own kernels, with no marker packets
+ * between the phases (the gaps between phases are in no phase).
  * Phases: 0 hash+sort, 1 reorder+ranges, 2 neighbours, 3 steering+integration,
  * 4 feature update.  With BRT_OPT_FUSED_STEP, phase 2 holds the fused
- * neighbours+steering kernel and phase 3 is empty.  brt_phase_times() synchronises the recorded events and
+ * neighbours+steering kernel and phase 3 is empty; a phase that launched no
+ * kernel reads 0.  brt_phase_times() synchronises the recorded events and
  * returns accumulated milliseconds per phase and the number of timed steps. */
 #define BRT_NUM_PHASES 5
 int brt_phase_timing_enable(brt_context* ctx, int enable);

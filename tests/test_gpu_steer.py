@@ -230,13 +230,36 @@ def test_full_stack_small_bit_exact(ctx, orc, shape):
 
 
 def test_phase_timing_reports_all_phases(ctx):
+    """Each phase's events are bound to its own kernels: the phases that ran
+    read > 0, the ones that launched nothing read 0, and in both step shapes
+    the phases together take no longer than the steps' wall time."""
+    import time
+
+    from behavert_amd.types import OPT_FUSED_STEP
+
     w = W.plain_4096()
-    ctx.phase_timing(True)
-    run_gpu_steps(ctx, w, 3)
-    ms, steps = ctx.phase_times()
-    ctx.phase_timing(False)
-    assert steps == 3
-    assert all(v >= 0 for v in ms.values()) and ms["neighbors"] > 0
+    old = ctx.get_option(OPT_FUSED_STEP)
+    try:
+        for fused in (1, 0):
+            ctx.set_option(OPT_FUSED_STEP, fused)
+            fl = Flock(ctx, w.grid, w.prox, w.steer, w.state)
+            fl.step(w.dt, 0)
+            torch.cuda.synchronize()
+            ctx.phase_timing(True)
+            t0 = time.perf_counter()
+            for i in range(3):
+                fl.step(w.dt, 1 + i)
+            torch.cuda.synchronize()
+            wall_ms = (time.perf_counter() - t0) * 1e3
+            ms, steps = ctx.phase_times()
+            ctx.phase_timing(False)
+            assert steps == 3
+            assert ms["hash_sort"] > 0 and ms["reorder_ranges"] > 0 and ms["neighbors"] > 0, ms
+            assert (ms["steer_integrate"] == 0) == bool(fused), ms
+            assert ms["features"] == 0, ms
+            assert sum(ms.values()) <= wall_ms, (ms, wall_ms)
+    finally:
+        ctx.set_option(OPT_FUSED_STEP, old)
 
 
 def test_error_paths_return_codes(ctx):
