@@ -111,13 +111,17 @@ int enqueue_step(brt_batch* b, const brt_steer_params* steer, float feature_rate
         std::swap(src, dst);
     }
     brt::ReorderArgs f{};
-    f.count = 3;
+    f.count = 2;
     f.in[0] = reinterpret_cast<const float4*>(buf->pos);
     f.out[0] = reinterpret_cast<float4*>(buf->sorted_pos);
     f.in[1] = reinterpret_cast<const float4*>(buf->forward);
     f.out[1] = reinterpret_cast<float4*>(buf->sorted_forward);
-    f.in[2] = reinterpret_cast<const float4*>(buf->smooth_accel);
-    f.out[2] = reinterpret_cast<float4*>(buf->sorted_smooth_accel);
+    if (buf->sorted_smooth_accel) {
+        // optional: without it the steering reads each agent's own by ID
+        f.in[2] = reinterpret_cast<const float4*>(buf->smooth_accel);
+        f.out[2] = reinterpret_cast<float4*>(buf->sorted_smooth_accel);
+        f.count = 3;
+    }
     // the LDS-list scan reads candidates from the packed x,y,z copy
     float* xyz = b->cfg.lds_lists ? b->xyz : nullptr;
     f.xyz0 = xyz;
@@ -154,7 +158,7 @@ int check_step_args(brt_batch* b, const brt_steer_params* steer, float feature_r
     BRT_REQUIRE(buf != nullptr, "buffers are NULL");
     BRT_REQUIRE(buf->n == b->total_n, "buf->n (%u) must equal the batch's total agents (%u)", buf->n, b->total_n);
     BRT_REQUIRE(buf->pos && buf->forward && buf->smooth_accel && buf->sorted_pos && buf->sorted_forward &&
-                    buf->sorted_smooth_accel && buf->hash && buf->cell_start && buf->cell_end && buf->lists,
+                    buf->hash && buf->cell_start && buf->cell_end && buf->lists,
                 "a required step buffer is NULL");
     BRT_REQUIRE(aligned16(buf->pos) && aligned16(buf->forward) && aligned16(buf->smooth_accel) &&
                     aligned16(buf->sorted_pos) && aligned16(buf->sorted_forward) &&

@@ -890,8 +890,8 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
     if ((rc = check_n_lists(ctx, b->n))) return rc;
     const uint32_t n = b->n;
     if (n == 0) return BRT_OK;
-    BRT_REQUIRE(b->pos && b->forward && b->smooth_accel && b->sorted_pos && b->sorted_forward &&
-                    b->sorted_smooth_accel && b->hash && b->cell_start && b->cell_end && b->lists,
+    BRT_REQUIRE(b->pos && b->forward && b->smooth_accel && b->sorted_pos && b->sorted_forward && b->hash &&
+                    b->cell_start && b->cell_end && b->lists,
                 "a required step buffer is NULL");
     BRT_REQUIRE(aligned16(b->pos) && aligned16(b->forward) && aligned16(b->smooth_accel) &&
                     aligned16(b->sorted_pos) && aligned16(b->sorted_forward) && aligned16(b->sorted_smooth_accel) &&
@@ -957,20 +957,24 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
         tmp = ctx->tmp;
     }
     // phase 1: reorder pos/forward/smooth_accel + cell ranges
+    // (the sorted smoothed acceleration only when the caller asked for it:
+    // without it the steering reads each agent's own by ID)
     brt::ReorderArgs f{};
-    f.count = 3;
-    f.in[0] = reinterpret_cast<const float4*>(b->pos);
-    f.out[0] = reinterpret_cast<float4*>(b->sorted_pos);
-    f.in[1] = reinterpret_cast<const float4*>(b->forward);
-    f.out[1] = reinterpret_cast<float4*>(b->sorted_forward);
-    f.in[2] = reinterpret_cast<const float4*>(b->smooth_accel);
-    f.out[2] = reinterpret_cast<float4*>(b->sorted_smooth_accel);
-    f.xyz0 = xyz;
-    if (sorted_features) {
-        f.count = 4;
-        f.in[3] = reinterpret_cast<const float4*>(b->features);
-        f.out[3] = reinterpret_cast<float4*>(tmp);
+    uint32_t nf = 0;
+    f.in[nf] = reinterpret_cast<const float4*>(b->pos);
+    f.out[nf++] = reinterpret_cast<float4*>(b->sorted_pos);
+    f.in[nf] = reinterpret_cast<const float4*>(b->forward);
+    f.out[nf++] = reinterpret_cast<float4*>(b->sorted_forward);
+    if (b->sorted_smooth_accel) {
+        f.in[nf] = reinterpret_cast<const float4*>(b->smooth_accel);
+        f.out[nf++] = reinterpret_cast<float4*>(b->sorted_smooth_accel);
     }
+    if (sorted_features) {
+        f.in[nf] = reinterpret_cast<const float4*>(b->features);
+        f.out[nf++] = reinterpret_cast<float4*>(tmp);
+    }
+    f.count = nf;
+    f.xyz0 = xyz;
     mark_phase(1);
     BRT_HIP(brt::launch_reorder_ranges(reinterpret_cast<const uint2*>(b->hash), n, f, grid->num_cells, b->cell_start,
                                        b->cell_end, s),

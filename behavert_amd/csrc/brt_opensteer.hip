@@ -96,7 +96,8 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
     if (id >= n) return;
     const float4 P = spos[i];
     const float4 Fw = sfwd[i];
-    const float4 SA = ssa[i];
+    // no sorted copy of the smoothed acceleration: the agent's own, by ID
+    const float4 SA = ssa ? ssa[i] : (APPLY ? o.sa_out[id] : make_float4(0.f, 0.f, 0.f, 0.f));
     const V3 myPos = xyz(P);
     const V3 myFwd = xyz(Fw);
 

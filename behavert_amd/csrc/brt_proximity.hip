@@ -135,7 +135,7 @@ struct ProxArgs {
     const float4* spos;
     const float* pxyz;  // optional: the sorted positions packed as x,y,z (12 bytes each)
     const float4* sfwd;
-    const float4* ssa;
+    const float4* ssa;  // sorted smooth_accel, or NULL: read by agent ID from o.sa_out
     const uint2* hash;
     const uint32_t* cell_start;
     const uint32_t* cell_end;
@@ -732,6 +732,20 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
         }
     }
 
+    // fused: the agent's own state, loaded before the list is finished so the
+    // loads (the smoothed acceleration by agent ID when there is no sorted
+    // copy) are in flight meanwhile
+    uint32_t own_id = 0u;
+    float4 own_fwd = make_float4(0.f, 0.f, 0.f, 0.f), own_sa = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (FUSE && alive) {
+        own_id = a.hash[L.i].y;
+        own_fwd = a.sfwd[L.i];
+        if (a.ssa)
+            own_sa = a.ssa[L.i];
+        else if (own_id < a.n)
+            own_sa = a.o.sa_out[own_id];
+    }
+
     if (LDSL && a.log_rounds) expand_log(L, a.K);
     if (LDSL || alive) finish_list<LDSL>(L, a.W);
     if (LDSL) __syncthreads();
@@ -753,10 +767,10 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
     // then integrate.  Same order and operations as k_steer over the written
     // list.
     if (FUSE && alive) {
-        const uint32_t id = a.hash[L.i].y;
+        const uint32_t id = own_id;
         if (id < a.n) {
-            const float4 Fw = a.sfwd[L.i];
-            const float4 SA = a.ssa[L.i];
+            const float4 Fw = own_fwd;
+            const float4 SA = own_sa;
             const V3 myPos = v3(P.x, P.y, P.z), myFwd = xyz(Fw);
             SteerAccum acc;
             acc.init();

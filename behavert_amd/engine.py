@@ -238,11 +238,16 @@ class Flock:
     EnvGrid3DPlugIn.h:157-173, Proximity3DPlugIn.h:167-174, OpenSteerWrapperPlugIn.h:153-178)."""
 
     def __init__(self, ctx: Context, grid: GridParams, prox: ProxParams, steer: SteerParams, state: dict,
-                 features: np.ndarray | None = None, wander: np.ndarray | None = None, device="cuda"):
+                 features: np.ndarray | None = None, wander: np.ndarray | None = None, device="cuda",
+                 sorted_smooth_accel: bool = True):
+        """sorted_smooth_accel=False: brt_step leaves sorted_smooth_accel out
+        (NULL) and reads each agent's own smoothed acceleration by ID; the
+        state, hash and lists are the same bits."""
         self.ctx = ctx
         self.grid = grid
         self.prox = prox
         self.steer = steer
+        self.keep_sorted_smooth_accel = bool(sorted_smooth_accel)
         _alloc_buffers(self, state, grid.num_cells, prox.max_neighbors, features, wander, device)
 
     def buffers(self) -> StepBuffers:
@@ -253,7 +258,7 @@ class Flock:
         b.smooth_accel = _ptr(self.smooth_accel)
         b.sorted_pos = _ptr(self.sorted_pos)
         b.sorted_forward = _ptr(self.sorted_forward)
-        b.sorted_smooth_accel = _ptr(self.sorted_smooth_accel)
+        b.sorted_smooth_accel = _ptr(self.sorted_smooth_accel) if self.keep_sorted_smooth_accel else None
         b.hash = _ptr(self.hash)
         b.cell_start = _ptr(self.cell_start)
         b.cell_end = _ptr(self.cell_end)
@@ -342,11 +347,13 @@ class Batch(Flock):
     [cell_bases[f], cell_bases[f] + grids[f].num_cells)."""
 
     def __init__(self, ctx: Context, grids, sizes, weights, prox: ProxParams, steer: SteerParams, state: dict,
-                 features: np.ndarray | None = None, wander: np.ndarray | None = None, device="cuda"):
+                 features: np.ndarray | None = None, wander: np.ndarray | None = None, device="cuda",
+                 sorted_smooth_accel: bool = True):
         self.ctx = ctx
         self.prox = prox
         self.steer = steer
         self.grid = None
+        self.keep_sorted_smooth_accel = bool(sorted_smooth_accel)
         F = len(grids)
         descs = (FlockDesc * F)()
         for f in range(F):

@@ -25,11 +25,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes_per_agent(w, packed=False) -> dict:
+def algorithmic_bytes_per_agent(w, packed=False, sorted_sa=False) -> dict:
     """Compulsory single-touch HBM bytes per agent for each phase of brt_step.
-    D digit passes, W list words, C/N cells per agent, F = 3 reordered float4s;
-    packed: the reorder also writes the 12-byte x,y,z copy the LDS-list scan
-    reads candidates from."""
+    D digit passes, W list words, C/N cells per agent, F reordered float4s
+    (position and heading, + the smoothed acceleration when its sorted copy
+    is kept; without it the steering reads the agent's own by ID, the same 16
+    bytes); packed: the reorder also writes the 12-byte x,y,z copy the
+    LDS-list scan reads candidates from."""
     from behavert_amd.types import list_stride
 
     n = w.n
@@ -42,7 +44,7 @@ def algorithmic_bytes_per_agent(w, packed=False) -> dict:
         # prepare (empty-cell fill 8 B/cell) + hash (R pos 16, W pair 8) + sort (R 8 hist, D x (R 8 + W 8))
         "hash_sort": 24 + (8 + 16 * D) + 8 * cpa,
         # R pair 8 (ID + key), 3 x (gather 16 + write 16) [+ W packed x,y,z 12]
-        "reorder_ranges": 8 + 32 * 3 + (12 if packed else 0),
+        "reorder_ranges": 8 + 32 * (3 if sorted_sa else 2) + (12 if packed else 0),
         # R own pos 16, W list 4W, R cell ranges 8 B/cell
         "neighbors": 16 + 4 * W + 8 * cpa,
         # R pos/fwd/sa 48 + list 4W + ID 4, W pos/fwd/sa 48 (neighbour gathers cache-resident)
@@ -67,20 +69,34 @@ PHASE_KERNELS = {"neighbors": "k_prox<false,false,*>", "steer_integrate": "k_ste
                  "features": "k_features", "neighbors_steer": "k_prox<true,false,*>"}
 
 
-def committed_traffic(phase: str, workload: str):
-    """HBM bytes per launch of the phase's kernels on the workload from the
-    committed rocprofv3 counter summaries (profiles/*_<workload>.md, a line
-    `traffic_bytes_per_launch[<phase>]: N` = FETCH_SIZE and WRITE_SIZE scaled
-    by the copy-kernel calibration), or None when there is none."""
+def committed_metric(name: str, phase: str, workload: str):
+    """A figure of the phase's kernels on the workload from the committed
+    rocprofv3 summaries (profiles/*_<workload>.md, a line `name[phase]: N`),
+    or None when there is none."""
     import glob
 
-    key = f"traffic_bytes_per_launch[{phase}]:"
+    key = f"{name}[{phase}]:"
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"*_{workload}.md"))):
         with open(path) as f:
             for line in f:
                 if line.startswith(key):
                     return float(line.split(":", 1)[1].split()[0])
     return None
+
+
+def committed_traffic(phase: str, workload: str):
+    """HBM bytes per launch of the phase's kernels (FETCH_SIZE and WRITE_SIZE
+    scaled by the copy-kernel calibration), or None."""
+    return committed_metric("traffic_bytes_per_launch", phase, workload)
+
+
+def committed_valu(phase: str, workload: str):
+    """The VALU roofline of the phase's kernel from the committed SQ counters:
+    VALU instructions per wave, issue fraction of the SIMDs' cycles, active
+    lanes per instruction, lane-ops against 256 CUs x 64 lanes x 2.4 GHz."""
+    keys = ("valu_insts_per_wave", "valu_issue_frac", "valu_active_lane_frac", "valu_lane_ops_frac_of_peak")
+    v = {k[5:]: committed_metric(k, phase, workload) for k in keys}
+    return v if all(x is not None for x in v.values()) else None
 
 
 def with_traffic(entry: dict, phase: str, workload: str, n: int) -> dict:
@@ -240,7 +256,8 @@ def full_stack(steps=20, warmup=3) -> dict:
 
     w = W.full_stack_4m()
     ctx = Context(0)
-    fl = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander)
+    fl = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander,
+               sorted_smooth_accel=False)
 
     def run(first, count):
         for i in range(first, first + count):
@@ -255,7 +272,7 @@ def full_stack(steps=20, warmup=3) -> dict:
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     fused, lds = ctx.step_shape(w.prox)
-    abytes = algorithmic_bytes_per_agent(w, packed=bool(lds) and not w.prox.use_knn)
+    abytes = algorithmic_bytes_per_agent(w, packed=bool(lds) and not w.prox.use_knn, sorted_sa=False)
     ph = phase_table(ctx, w, abytes, fused=fused)
     ctx.phase_timing(False)
     ctx.close()
@@ -264,7 +281,8 @@ def full_stack(steps=20, warmup=3) -> dict:
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ph[dom]["frac"],
                 "traffic": committed_traffic(dom, w.name), "bytes_per_agent": ph[dom]["bytes_per_agent"],
                 "agents_per_launch": w.n, "avg_launch_ms": ph[dom]["ms"]}
-    return {**w.describe(), "ms_per_step": round(1e3 * el / steps, 4), "agent_updates_per_s": w.n * steps / el,
+    return {**w.describe(), "sorted_smooth_accel": False, "ms_per_step": round(1e3 * el / steps, 4),
+            "agent_updates_per_s": w.n * steps / el,
             "phases": ph, "roofline": roofline, "timed_steps": steps, "step_shape": {"fused": fused, "lds_lists": lds}}
 
 
@@ -398,7 +416,7 @@ def consumers(reps=20) -> dict:
     return {"workload": w.name, "n": n, "K": K, "reps": reps, "passes": res}
 
 
-def batched(steps=20, loop_steps=3) -> dict:
+def batched(steps=20, loop_steps=3, config: str | None = None) -> dict:
     """256 flocks (sizes log-uniform in [1024, 65536], 0.5 agents per unit^3, K 32,
     per-flock weights): the autotuned, graph-replayed batched step vs brt_step
     looped over the same flocks."""
@@ -412,9 +430,15 @@ def batched(steps=20, loop_steps=3) -> dict:
 
     w = W.batched_flocks()
     ctx = Context(0)
-    b = Batch(ctx, w.grids, w.sizes, w.weights, w.prox, w.steer, w.concat_state())
+    b = Batch(ctx, w.grids, w.sizes, w.weights, w.prox, w.steer, w.concat_state(), sorted_smooth_accel=False)
     t0 = time.perf_counter()
-    cfg = b.autotune(w.dt, 2)
+    if config:
+        # a fixed launch configuration "key=value,..." instead of the tuner
+        # (counter passes see the same kernels as the traced run)
+        b.set_config(**{k: int(v) for k, v in (kv.split("=") for kv in config.split(","))})
+        cfg = b.config()
+    else:
+        cfg = b.autotune(w.dt, 2)
     tune_s = time.perf_counter() - t0
 
     def timed(fn, k):
@@ -432,7 +456,7 @@ def batched(steps=20, loop_steps=3) -> dict:
         sp = SteerParams()
         C.memmove(C.byref(sp), C.byref(w.steer), C.sizeof(SteerParams))
         sp.separation[0], sp.alignment[0], sp.cohesion[0] = (float(x) for x in w.weights[f])
-        flocks.append(Flock(ctx, w.grids[f], w.prox, sp, w.states[f]))
+        flocks.append(Flock(ctx, w.grids[f], w.prox, sp, w.states[f], sorted_smooth_accel=False))
 
     def loop(i):
         for fl in flocks:
@@ -450,9 +474,9 @@ def batched(steps=20, loop_steps=3) -> dict:
     D = (max(1, int(cells - 1).bit_length()) + 7) // 8
     Wl, cpa = list_stride(w.prox.max_neighbors), cells / w.n
     packed = 12 if cfg.lds_lists else 0
-    bpa = 24 + (8 + 16 * D) + 8 * cpa + (8 + 32 * 3 + packed) + (16 + 4 * Wl + 8 * cpa + 36 + 48)
+    bpa = 24 + (8 + 16 * D) + 8 * cpa + (8 + 32 * 2 + packed) + (16 + 4 * Wl + 8 * cpa + 36 + 48)
     gbs = bpa * w.n / 1e9 / (batch_ms / 1e3)
-    return {**w.describe(), "tuned": cfg.as_dict(), "tune_s": round(tune_s, 3),
+    return {**w.describe(), "sorted_smooth_accel": False, "tuned": cfg.as_dict(), "tune_s": round(tune_s, 3),
             "roofline": {"bound": "hbm", "scope": "batched step", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_agent": round(bpa, 2),
                          "traffic": committed_traffic("batched", "batched_256")},
@@ -485,16 +509,19 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and batched measurements")
     ap.add_argument("--only", choices=("adjoint", "batched", "full_stack", "clustering", "consumers"),
                     help="run one extra section alone and print its JSON (for profiling)")
+    ap.add_argument("--batch-config", default=None,
+                    help="fixed batched-step configuration, e.g. prox_block=256,uniform_max_cells=1 (skips the tuner)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with three 256 MiB device copies (the reference kernel for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU: the step path has no multi-GPU form "
-                                   "(see DESIGN.md, 'How the path would shard')", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the step path runs on exactly one GPU: it has no multi-GPU form",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
     if args.only:
-        print(json.dumps({args.only: {"adjoint": adjoint_ratio, "batched": batched,
+        print(json.dumps({args.only: {"adjoint": adjoint_ratio,
+                                      "batched": lambda: batched(config=args.batch_config),
                                       "full_stack": full_stack, "clustering": clustering,
                                       "consumers": consumers}[args.only]()}),
               flush=True)
@@ -509,7 +536,8 @@ def main():
 
     w = W.WORKLOADS[args.workload]()
     ctx = Context(0)
-    fl = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander)
+    fl = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander,
+               sorted_smooth_accel=False)
 
     def run(first, count):
         for i in range(first, first + count):
@@ -517,7 +545,8 @@ def main():
                 fl.steer.set_targets(w.targets_at(i))
             fl.step(w.dt, i, w.feature_rate)
 
-    abytes = algorithmic_bytes_per_agent(w, packed=bool(ctx.step_shape(w.prox)[1]) and not w.prox.use_knn)
+    abytes = algorithmic_bytes_per_agent(w, packed=bool(ctx.step_shape(w.prox)[1]) and not w.prox.use_knn,
+                                         sorted_sa=False)
     run(0, args.warmup)
     torch.cuda.synchronize()
     # the timed steps: nothing between them but the steps themselves
@@ -562,7 +591,7 @@ def main():
         dom = max((k for k in fz if isinstance(fz[k], dict) and abytes.get(k, 0) > 0), key=lambda k: fz[k]["ms"])
         roofline = {"bound": "hbm", "kernel": fz[dom]["kernel"], "phase": dom, "achieved": fz[dom]["GB/s"],
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fz[dom]["frac"],
-                    "traffic": committed_traffic(dom, w.name),
+                    "traffic": committed_traffic(dom, w.name), "valu": committed_valu(dom, w.name),
                     "bytes_per_agent": round(abytes[dom], 2), "agents_per_launch": w.n,
                     "avg_launch_ms": fz[dom]["ms"], "timed_steps": fz["steps"]}
         uf = phases["unfused_step"]
@@ -600,7 +629,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": w.describe(),
+        "config": {**w.describe(), "sorted_smooth_accel": False},
         "hbm_frac_step": round(step_gbs / HBM_PEAK_GBS, 4),
         "roofline": roofline,
         "phases": phases,

@@ -430,7 +430,11 @@ int brt_bc_feature_update(brt_context* ctx, const brt_prox_params* prox,
 /* Device buffers of one flock.  pos/forward/smooth_accel (by ID) are read and
  * overwritten with the new state.  The sorted copies, hash, cell ranges and lists
  * are outputs of the step (the Proximity3D/EnvGrid3D features of the reference).
- * features/features_out and wander are optional (NULL disables). */
+ * features/features_out and wander are optional (NULL disables).
+ * sorted_smooth_accel is optional too: with NULL the step does not gather the
+ * smoothed acceleration into sorted order (the reference reorders it only for
+ * the integrator, EnvGrid3DPlugIn.h:333-411) and the integrator reads each
+ * agent's own by ID instead; the new state, hash and lists are the same bits. */
 typedef struct brt_step_buffers {
     uint32_t n;
     float* pos;

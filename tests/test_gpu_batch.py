@@ -176,6 +176,33 @@ def test_graph_replay_matches_direct_launches(ctx):
         assert first_mismatch(a.view(np.uint32), c.view(np.uint32)) is None
 
 
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_batch_without_sorted_smooth_accel_is_the_same(ctx, use_graph):
+    """A batch stepped with sorted_smooth_accel NULL (own smoothed acceleration
+    read by ID) equals the one with the sorted copy, bit for bit, direct and
+    graph-replayed, with wander and features."""
+    sizes, grids, weights, states, prox = small_batch(seed=6, sizes=(1500, 300, 2200), K=32)
+    steer = W.plain_steer()
+    steer.wander_weight = 0.3
+    steer.wander_seed = 5
+    g = W.rng(6)
+    wander = np.zeros((int(sizes.sum()), 2), np.float32)
+    feats = g.random((int(sizes.sum()), 4)).astype(np.float32)
+    outs = []
+    for keep in (True, False):
+        b = Batch(ctx, grids, sizes, weights, prox, steer, concat(states), features=feats, wander=wander,
+                  sorted_smooth_accel=keep)
+        b.set_config(use_graph=use_graph)
+        for t in range(4):
+            b.step(0.05, t, 0.1)
+            torch.cuda.synchronize()
+        outs.append([getattr(b, k).cpu().numpy().copy() for k in
+                     ("pos", "forward", "smooth_accel", "features", "wander", "hash", "lists")])
+        b.close()
+    for k, (a, c) in enumerate(zip(*outs)):
+        assert first_mismatch(a.view(np.uint32), c.view(np.uint32)) is None, k
+
+
 def test_autotune_restores_state_and_caches(ctx, tmp_path):
     sizes, grids, weights, states, prox = small_batch(seed=5, sizes=(3000, 1200, 4000, 64))
     b = Batch(ctx, grids, sizes, weights, prox, W.plain_steer(), concat(states))

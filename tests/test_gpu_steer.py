@@ -374,3 +374,41 @@ def test_neighbourhood_cuts_on_exact_boundaries(ctx, orc, cos_max, min_r, radius
                 assert first_mismatch(got[k].view(np.uint32), ref[k].view(np.uint32)) is None, (fused, k)
     finally:
         ctx.set_option(OPT_FUSED_STEP, old)
+
+
+@pytest.mark.parametrize("case", ["blobs_1m", "features_wander"])
+@pytest.mark.parametrize("fused", [1, 0])
+def test_step_without_sorted_smooth_accel_is_the_same(ctx, case, fused):
+    """brt_step with sorted_smooth_accel NULL (the integrator reads each agent's
+    own smoothed acceleration by ID, the reorder gathers one feature less)
+    gives the same state, hash, lists and features, bit for bit, over 3 steps,
+    in the fused and the two-pass step."""
+    from behavert_amd.types import OPT_FUSED_STEP
+    if case == "features_wander":
+        w = W.full_stack_4m(n=1 << 16)
+        w.side = 640.0 * (1 / 64) ** (1 / 3)
+        w.state = W.uniform_flock(w.n, w.side, 0)
+        w.grid = W.cube_grid(w.side)
+        w.moving_targets = False
+    else:
+        w = W.blobs_1m()
+    old = ctx.get_option(OPT_FUSED_STEP)
+    runs = []
+    try:
+        ctx.set_option(OPT_FUSED_STEP, fused)
+        for keep in (True, False):
+            fl = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander,
+                       sorted_smooth_accel=keep)
+            for t in range(3):
+                fl.step(w.dt, t, w.feature_rate)
+            torch.cuda.synchronize()
+            st, go = fl.state(), fl.grid_outputs()
+            runs.append(({k: v for k, v in st.items() if v is not None}, go))
+            del fl
+    finally:
+        ctx.set_option(OPT_FUSED_STEP, old)
+    (a, ga), (b, gb) = runs
+    for k in a:
+        assert first_mismatch(b[k].view(np.uint32), a[k].view(np.uint32)) is None, k
+    for k in ("hash", "lists", "cell_start", "cell_end", "sorted_pos", "sorted_forward"):
+        assert first_mismatch(gb[k].view(np.uint32), ga[k].view(np.uint32)) is None, k
