@@ -157,11 +157,26 @@ def cpu_baseline(w, sample_steps: int) -> dict:
     for i in range(sample_steps):
         hf.step(w.steer, w.dt, 1 + i, w.feature_rate)
     dt = time.perf_counter() - t0
-    return {"value": w.n * sample_steps / dt, "unit": "agent-updates/s", "cores": oracle.num_threads(),
-            "cpu_model": cpu_model(), "kind": "port",
-            "sample": f"{sample_steps} steps of the {w.name} workload ({w.n} agents) after 1 untimed step, "
-                      f"FP32 C++ oracle (hash, LSD radix sort, gather, ranges, 27-cell lists, steering), OpenMP",
-            "ms_per_step": 1e3 * dt / sample_steps}
+    out = {"value": w.n * sample_steps / dt, "unit": "agent-updates/s", "cores": oracle.num_threads(),
+           "cpu_model": cpu_model(), "kind": "port",
+           "sample": f"{sample_steps} steps of the {w.name} workload ({w.n} agents) after 1 untimed step, "
+                     f"FP32 C++ oracle (hash, LSD radix sort, gather, ranges, 27-cell lists, steering), OpenMP",
+           "ms_per_step": 1e3 * dt / sample_steps}
+    # the reference tree's own CPU proximity code (OpenSteer's lq.c bin
+    # lattice: uncapped sphere queries, no canonical order) on the same
+    # positions: a reference-code figure for the neighbour search alone
+    stride = 16
+    lq = oracle.lq_flock_baseline(w.state["pos"], w.grid, w.prox.search_radius, threads, stride)
+    if lq is not None:
+        out["reference_neighbour_search"] = {
+            "value": lq["queries"] / lq["query_s"], "unit": "neighbourhood queries/s", "cores": threads,
+            "kind": "reference",
+            "code": "Samples/OpensteerDemo/src/lq.c (OpenSteer locality-query bin lattice, built into oracle/_ref)",
+            "build_ms": round(1e3 * lq["build_s"], 3), "query_ms": round(1e3 * lq["query_s"], 3),
+            "neighbours_per_query": round(lq["neighbours"] / lq["queries"], 2),
+            "sample": f"all {w.n} agents of the {w.name} initial state in the lattice (the grid's box and cells), "
+                      f"one radius-{w.prox.search_radius} query for every {stride}th ({lq['queries']} queries)"}
+    return out
 
 
 def adjoint_ratio(steps=None, every=None) -> dict:

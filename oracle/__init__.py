@@ -26,6 +26,12 @@ restates.  Parity pins (tests/test_oracle_*.py):
                      hand-derived KATs (tests/test_oracle_consumers.py)
   snapshot.py ...... numpy restatement of the snapshot format; FNV-1a 64 test
                      vectors (tests/test_snapshot.py)
+
+oracle/_ref/liblq_ref.so (oracle/build_ref.sh, when the reference checkout is
+present) is the reference tree's own OpenSteer locality-query database
+(Samples/OpensteerDemo/src/lq.c) with a timing driver (oracle/lq_driver.c):
+lq_flock_baseline() times it as the reference-code CPU neighbour search; its
+counts are pinned against a brute-force count (tests/test_ref_lq.py).
 """
 from __future__ import annotations
 
@@ -43,8 +49,40 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 _lib = None
 
 
+REF_LQ_PATH = os.path.join(HERE, "_ref", "liblq_ref.so")
+
+
 def build():
     subprocess.run(["bash", os.path.join(HERE, "build.sh")], check=True)
+    # the reference's own lq.c, only where the reference checkout is present
+    subprocess.run(["bash", os.path.join(HERE, "build_ref.sh")], check=True)
+
+
+def lq_flock_baseline(pos4: np.ndarray, grid, radius: float, threads: int, stride: int = 1) -> dict | None:
+    """OpenSteer's locality-query database (the reference tree's lq.c) over a
+    flock: database build (one lqUpdateForNewLocation per agent, serial) and
+    one sphere query (lqMapOverAllObjectsInLocality, `threads` OpenMP threads)
+    for every stride-th agent.  The lattice is the grid's box and cells.
+    None when oracle/_ref was not built."""
+    if not os.path.exists(REF_LQ_PATH):
+        return None
+    L = C.CDLL(REF_LQ_PATH)
+    f = L.lq_flock_baseline
+    f.restype = C.c_int
+    F3, I3 = C.c_float * 3, C.c_int * 3
+    f.argtypes = [C.c_void_p, C.c_uint32, F3, F3, I3, C.c_float, C.c_uint32, C.c_int, C.POINTER(C.c_double),
+                  C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    gs = [int(x) for x in grid.grid_size]
+    origin = F3(*[float(x) for x in grid.world_origin])
+    size = F3(*[float(grid.cell_size[a]) * gs[a] for a in range(3)])
+    b, q, nb = C.c_double(), C.c_double(), C.c_uint64()
+    rc = f(pos4.ctypes.data, pos4.shape[0], origin, size, I3(*gs), float(radius), int(stride), int(threads),
+           C.byref(b), C.byref(q), C.byref(nb))
+    if rc != 0:
+        raise MemoryError("lq_flock_baseline: out of memory")
+    queries = (pos4.shape[0] + stride - 1) // stride
+    return {"build_s": b.value, "query_s": q.value, "neighbours": int(nb.value), "queries": queries}
 
 
 def lib():
