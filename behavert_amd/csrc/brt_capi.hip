@@ -201,7 +201,7 @@ const char* brt_version(void) { return "behavert_amd 0.1 (gfx950)"; }
 // sort metadata, a 16-byte-per-agent buffer and the packed positions, sized for
 // BRT_NOALLOC_MAX_AGENTS agents (about 30 MB).  Host code reaches it through
 // the address hipGetSymbolAddress returns on each device, kept in the context.
-constexpr size_t kNoAllocMetaWords = 1088 + 4 * ((BRT_NOALLOC_MAX_AGENTS + 2047) / 2048) * 256;
+constexpr size_t kNoAllocMetaWords = brt::kMetaStatus + 4 * ((BRT_NOALLOC_MAX_AGENTS + 2047) / 2048) * 256;
 constexpr size_t kNoAllocMetaOffset = 0;
 constexpr size_t kNoAllocTmpOffset = (kNoAllocMetaOffset + kNoAllocMetaWords * 4 + 255) / 256 * 256;
 constexpr size_t kNoAllocXyzOffset = (kNoAllocTmpOffset + (size_t)BRT_NOALLOC_MAX_AGENTS * 16 + 255) / 256 * 256;

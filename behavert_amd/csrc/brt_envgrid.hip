@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(256) k_hash_hist(brt_grid_params g, const floa
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) {
         uint32_t v = s_hist[t];
-        if (v) atomicAdd(&hist[t], v);
+        if (v) atomicAdd(&hist[(blockIdx.x % kHistCopies) * kHistWords + t], v);
     }
 }
 
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_batch_hash_hist(BatchView b, const floa
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) {
         uint32_t v = s_hist[t];
-        if (v) atomicAdd(&hist[t], v);
+        if (v) atomicAdd(&hist[(blockIdx.x % kHistCopies) * kHistWords + t], v);
     }
 }
 
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) k_hist(const uint2* __restrict__ in, uint
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) {
         uint32_t v = s_hist[t];
-        if (v) atomicAdd(&hist[t], v);
+        if (v) atomicAdd(&hist[(blockIdx.x % kHistCopies) * kHistWords + t], v);
     }
 }
 
@@ -157,6 +157,7 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
     __shared__ uint32_t s_whist[4][kRadix];
     __shared__ uint32_t s_digit_start[kRadix];
     __shared__ uint32_t s_base[kRadix];
+    __shared__ uint32_t s_gcount[kRadix];
     __shared__ uint32_t s_tmp[4];
     __shared__ uint2 s_items[TILE];
 
@@ -178,6 +179,15 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
     for (int k = 0; k < ITEMS; ++k) {
         uint32_t idx = wbase + k * 64 + lane;
         it[k] = idx < n ? in[idx] : make_uint2(0u, 0u);
+    }
+    // this pass's global count of digit tid, summed over the histogram copies
+    // while the items load, and parked in LDS (not in registers) over the
+    // ranking
+    {
+        uint32_t gc = 0u;
+#pragma unroll
+        for (uint32_t c = 0; c < kHistCopies; ++c) gc += ghist[c * kHistWords + tid];
+        s_gcount[tid] = gc;
     }
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
@@ -224,7 +234,7 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
         __hip_atomic_store(my_status, kFlagAgg | tcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     const uint32_t digit_start = block_excl_scan_256(tcount, s_tmp);
-    const uint32_t gex = block_excl_scan_256(ghist[tid], s_tmp);
+    const uint32_t gex = block_excl_scan_256(s_gcount[tid], s_tmp);
     s_digit_start[tid] = digit_start;
 
     // decoupled look-back over the preceding tiles for digit tid
@@ -317,14 +327,15 @@ __global__ void __launch_bounds__(256) k_reorder_ranges(const uint2* __restrict_
 }
 
 // Workgroups of the histogram kernels: each adds its per-workgroup histograms
-// of every pass into the global ones with one atomic per nonzero bin, so fewer
-// workgroups mean fewer contended global atomics but more keys per thread.
-// One workgroup per 4096 keys, 256 to 1024 of them, measured fastest on gfx950
-// (1M agents: 12.5 us with 256 against 24.6 us with 1024; 4M agents: 1024).
-inline uint32_t hist_blocks(uint32_t n) {
-    const uint32_t b = n / 4096u;
-    return b < 256u ? 256u : (b > 1024u ? 1024u : b);
-}
+// of every pass into one of the kHistCopies global copies, one atomic per
+// nonzero bin.  With a single copy the contended atomics set the kernel's time
+// (1M agents: 12.5 us with 256 workgroups against 24.6 us with 1024); spread
+// over 8 copies they no longer do, and 1024 workgroups (capped at one per
+// 256 keys) measured fastest (1M agents: 8.6 us; 4M: 24.8 us, from 38.7).
+#ifndef BRT_HIST_BLOCKS
+#define BRT_HIST_BLOCKS 1024u
+#endif
+inline uint32_t hist_blocks(uint32_t) { return BRT_HIST_BLOCKS; }
 
 inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t cap) {
     uint32_t g = (n + block - 1) / block;

@@ -38,9 +38,17 @@ constexpr int kSortItems = 16;               // keys per thread
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadix = 256;                  // 8-bit digits
 constexpr int kMaxPasses = 4;
-constexpr uint32_t kMetaHist = 0;            // uint32 hist[4][256]
-constexpr uint32_t kMetaCounters = 1024;     // uint32 tile counters[4] (padded to 64)
-constexpr uint32_t kMetaStatus = 1088;       // uint32 status[passes][tiles][256]
+// The digit histograms are accumulated in kHistCopies copies (a histogram
+// workgroup adds into copy blockIdx % kHistCopies), so that each bin's global
+// atomics are spread over that many addresses; a sort pass sums the copies.
+#ifndef BRT_HIST_COPIES
+#define BRT_HIST_COPIES 8u
+#endif
+constexpr uint32_t kHistCopies = BRT_HIST_COPIES;
+constexpr uint32_t kHistWords = kMaxPasses * kRadix;        // one copy: hist[4][256]
+constexpr uint32_t kMetaHist = 0;                           // uint32 hist[copies][4][256]
+constexpr uint32_t kMetaCounters = kHistCopies * kHistWords;  // uint32 tile counters[4] (padded to 64)
+constexpr uint32_t kMetaStatus = kMetaCounters + 64;          // uint32 status[passes][tiles][256]
 
 struct SortPlan {
     uint32_t passes;
