@@ -9,6 +9,10 @@ lists recorded (the step is deterministic, so the records equal the
 forward's), then the adjoint kernels walk the records backwards.  The sort and
 the lists are thus held fixed as the forward built them, and the adjoint has
 no atomics, so the gradients do not depend on the checkpoint interval.
+The replay of a segment does not depend on the adjoint of the later one, so
+(overlap=True) segment s-1 is replayed on a second stream, with a second
+context and a second set of records, while the adjoint walks segment s: the
+kernels of both are small at these sizes and share the GPU.
 
 Gradients are returned for w (3 behaviour weights), pos0, forward0 (xyz = unit
 heading, w = speed) and smooth_accel0.  Differentiating with respect to an
@@ -34,7 +38,7 @@ class Rollout:
     """Buffers and settings of one differentiable rollout configuration."""
 
     def __init__(self, ctx: Context, grid: GridParams, prox: ProxParams, steer: SteerParams, n: int, dt: float,
-                 steps: int, checkpoint_every: int, device="cuda"):
+                 steps: int, checkpoint_every: int, device="cuda", overlap: bool = True):
         if steer.use_2d_projection or steer.wander_weight != 0.0:
             raise ValueError("the adjoint supports neither 2-D projection nor wander")
         self.ctx, self.grid, self.prox, self.n, self.dt = ctx, grid, prox, n, float(dt)
@@ -53,14 +57,20 @@ class Rollout:
         self.lists = torch.empty((n, self.W), **i32)
         self.cell_start = torch.empty((grid.num_cells,), **i32)
         self.cell_end = torch.empty((grid.num_cells,), **i32)
-        # per-step records of one segment
-        self.r_spos, self.r_sfwd, self.r_ssa = f4(self.K, n), f4(self.K, n), f4(self.K, n)
-        self.r_hash = torch.empty((self.K, n, 2), **i32)
-        self.r_lists = torch.empty((self.K, n, self.W), **i32)
+        # per-step records of one segment (two sets when replays overlap the
+        # adjoint: segment s-1 is recorded while segment s is read)
+        self.overlap = bool(overlap)
+        S = 2 if self.overlap else 1
+        self.r_spos, self.r_sfwd, self.r_ssa = f4(S, self.K, n), f4(S, self.K, n), f4(S, self.K, n)
+        self.r_hash = torch.empty((S, self.K, n, 2), **i32)
+        self.r_lists = torch.empty((S, self.K, n, self.W), **i32)
         # cell ranges, rows padded to 16 bytes (the step's alignment)
         cpad = (grid.num_cells + 3) // 4 * 4
-        self.r_cs = torch.empty((self.K, cpad), **i32)
-        self.r_ce = torch.empty((self.K, cpad), **i32)
+        self.r_cs = torch.empty((S, self.K, cpad), **i32)
+        self.r_ce = torch.empty((S, self.K, cpad), **i32)
+        # the replays' own context (scratch) and stream
+        self.ctx_replay = Context(ctx.device) if self.overlap else ctx
+        self.stream_replay = torch.cuda.Stream(device=dev) if self.overlap else None
         # adjoints (two sets, ping-pong) and the weight gradient
         self.g = [(f4(n), f4(n), f4(n)), (f4(n), f4(n), f4(n))]
         self.wgrad = torch.zeros(3, dtype=torch.float32, device=dev)
@@ -68,8 +78,9 @@ class Rollout:
         # argument blocks built once: the per-step calls only pass them on
         self._bufs = self._buffers(self.spos, self.sfwd, self.ssa, self.hash, self.lists, self.cell_start,
                                    self.cell_end)
-        self._rec_bufs = [self._buffers(self.r_spos[k], self.r_sfwd[k], self.r_ssa[k], self.r_hash[k],
-                                        self.r_lists[k], self.r_cs[k], self.r_ce[k]) for k in range(self.K)]
+        self._rec_bufs = [[self._buffers(self.r_spos[q, k], self.r_sfwd[q, k], self.r_ssa[q, k], self.r_hash[q, k],
+                                         self.r_lists[q, k], self.r_cs[q, k], self.r_ce[q, k]) for k in range(self.K)]
+                          for q in range(S)]
         self._adj_args = {}
 
     def set_weights(self, w):
@@ -85,21 +96,33 @@ class Rollout:
         b.cell_start, b.cell_end = _p(cell_start), _p(cell_end)
         return b
 
-    def _step(self, t, bufs: StepBuffers):
-        self.ctx.check(lib().brt_step(self.ctx.handle, C.byref(self.grid), C.byref(self.prox), C.byref(self.steer),
-                                      0.0, self.dt, int(t), C.byref(bufs), current_stream()))
+    def _step(self, t, bufs: StepBuffers, ctx: Context | None = None):
+        ctx = self.ctx if ctx is None else ctx
+        ctx.check(lib().brt_step(ctx.handle, C.byref(self.grid), C.byref(self.prox), C.byref(self.steer),
+                                 0.0, self.dt, int(t), C.byref(bufs), current_stream()))
 
-    def _adjoint(self, k, cur):
-        key = (k, cur)
+    def _adjoint(self, q, k, cur):
+        key = (q, k, cur)
         args = self._adj_args.get(key)
         if args is None:
             go, gi = self.g[cur], self.g[1 - cur]
             args = (self.ctx.handle, C.byref(self.grid), C.byref(self.prox), C.byref(self.steer), self.dt, self.n,
-                    _p(self.r_spos[k]), _p(self.r_sfwd[k]), _p(self.r_ssa[k]), _p(self.r_hash[k]),
-                    _p(self.r_lists[k]), _p(self.r_cs[k]), _p(self.r_ce[k]), _p(go[0]), _p(go[1]), _p(go[2]),
-                    _p(gi[0]), _p(gi[1]), _p(gi[2]), _p(self.wgrad))
+                    _p(self.r_spos[q, k]), _p(self.r_sfwd[q, k]), _p(self.r_ssa[q, k]), _p(self.r_hash[q, k]),
+                    _p(self.r_lists[q, k]), _p(self.r_cs[q, k]), _p(self.r_ce[q, k]), _p(go[0]), _p(go[1]),
+                    _p(go[2]), _p(gi[0]), _p(gi[1]), _p(gi[2]), _p(self.wgrad))
             self._adj_args[key] = args
         self.ctx.check(lib().brt_osw_adjoint_step(*args, current_stream()))
+
+    def _replay(self, seg, q, ctx):
+        """Re-runs the segment (t0, pos, fwd, sa) from its checkpoint into record set q."""
+        t0, p, f, a = seg
+        t1 = min(t0 + self.K, self.steps)
+        self.pos.copy_(p)
+        self.fwd.copy_(f)
+        self.sa.copy_(a)
+        for t in range(t0, t1):
+            self._step(t, self._rec_bufs[q][t - t0], ctx)
+        return t0, t1
 
     def forward(self, weights, pos0, fwd0, sa0):
         """Runs the rollout; keeps checkpoints; returns the final (pos, forward, smooth_accel)."""
@@ -127,16 +150,40 @@ class Rollout:
         self.g[cur][1].copy_(gfwd)
         self.g[cur][2].copy_(gsa)
         self.wgrad.zero_()
-        for (t0, p, f, a) in reversed(checkpoints):
-            t1 = min(t0 + self.K, self.steps)
-            self.pos.copy_(p)
-            self.fwd.copy_(f)
-            self.sa.copy_(a)
-            for t in range(t0, t1):
-                self._step(t, self._rec_bufs[t - t0])
-            for t in range(t1 - 1, t0 - 1, -1):
-                self._adjoint(t - t0, cur)
-                cur = 1 - cur
+        segs = list(reversed(checkpoints))
+        if not self.overlap:
+            for seg in segs:
+                t0, t1 = self._replay(seg, 0, self.ctx)
+                for t in range(t1 - 1, t0 - 1, -1):
+                    self._adjoint(0, t - t0, cur)
+                    cur = 1 - cur
+        else:
+            main, side = torch.cuda.current_stream(), self.stream_replay
+            side.wait_stream(main)  # the checkpoints and the working state
+            replayed, adjointed, spans = [], [], []
+
+            def replay(i):
+                with torch.cuda.stream(side):
+                    if i >= 2:
+                        side.wait_event(adjointed[i - 2])  # record set i % 2 was read by adjoint i - 2
+                    spans.append(self._replay(segs[i], i % 2, self.ctx_replay))
+                    ev = torch.cuda.Event()
+                    ev.record(side)
+                    replayed.append(ev)
+
+            replay(0)
+            for i in range(len(segs)):
+                if i + 1 < len(segs):
+                    replay(i + 1)  # enqueued before this segment's adjoint, so the two overlap
+                main.wait_event(replayed[i])
+                t0, t1 = spans[i]
+                for t in range(t1 - 1, t0 - 1, -1):
+                    self._adjoint(i % 2, t - t0, cur)
+                    cur = 1 - cur
+                ev = torch.cuda.Event()
+                ev.record(main)
+                adjointed.append(ev)
+            main.wait_stream(side)  # leave no replay work pending on the side stream
         g = self.g[cur]
         return self.wgrad.clone(), g[0].clone(), g[1].clone(), g[2].clone()
 

@@ -229,9 +229,10 @@ def adjoint_ratio(steps=None, every=None) -> dict:
     for _ in range(reps):
         go, gi = ro.g[0], ro.g[1]
         ctx.check(L.brt_osw_adjoint_step(ctx.handle, C.byref(ro.grid), C.byref(ro.prox), C.byref(ro.steer), ro.dt,
-                                         ro.n, ro.r_spos[0].data_ptr(), ro.r_sfwd[0].data_ptr(),
-                                         ro.r_ssa[0].data_ptr(), ro.r_hash[0].data_ptr(), ro.r_lists[0].data_ptr(),
-                                         ro.r_cs[0].data_ptr(), ro.r_ce[0].data_ptr(), go[0].data_ptr(),
+                                         ro.n, ro.r_spos[0, 0].data_ptr(), ro.r_sfwd[0, 0].data_ptr(),
+                                         ro.r_ssa[0, 0].data_ptr(), ro.r_hash[0, 0].data_ptr(),
+                                         ro.r_lists[0, 0].data_ptr(), ro.r_cs[0, 0].data_ptr(),
+                                         ro.r_ce[0, 0].data_ptr(), go[0].data_ptr(),
                                          go[1].data_ptr(), go[2].data_ptr(), gi[0].data_ptr(), gi[1].data_ptr(),
                                          gi[2].data_ptr(), ro.wgrad.data_ptr(),
                                          torch.cuda.current_stream().cuda_stream))
@@ -251,6 +252,7 @@ def adjoint_ratio(steps=None, every=None) -> dict:
     adj_bpa = (200 + 4 * Wl) + (104 + 4 * Wl)
     adj_gbs = adj_bpa * w.n / 1e9 / (adj_ms / 1e3)
     return {"workload": "differentiable_65k", "n": w.n, "steps": T, "checkpoint_every": K,
+            "replay_overlaps_adjoint": ro.overlap,
             "forward_step_ms": round(step_ms, 5), "adjoint_step_ms": round(adj_ms, 5),
             "step_ratio": round((step_ms + adj_ms) / step_ms, 3),
             "forward_rollout_ms": round(fwd_ms, 3), "backward_rollout_ms": round(bwd_ms, 3),

@@ -389,3 +389,27 @@ def test_two_forwards_then_backward_of_the_first(ctx, orc):
     g_shared = grads(ro, [start, other])
     np.testing.assert_allclose(g_shared[0], g_alone[0], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(g_shared[1], g_alone[1], rtol=1e-4, atol=1e-7 * np.abs(g_alone[1]).max())
+
+
+def test_overlapped_replay_gives_the_same_gradients(ctx):
+    """The backward with each segment's replay on a second stream and context,
+    overlapping the adjoint of the later segment (the default), returns the
+    same bits as the serial backward, with checkpoints every 7 of 40 steps (a
+    short last segment) and twice in a row."""
+    w = small_flock(5, 3000, 40.0, K=32)
+    pos0, fw0, sa0 = dev(w.state["pos"]), dev(w.state["forward"]), dev(w.state["smooth_accel"])
+    rng = np.random.default_rng(2)
+    gp = np.zeros((w.n, 4), np.float32)
+    gp[:, :3] = rng.standard_normal((w.n, 3)) / w.n
+    z = torch.zeros((w.n, 4), dtype=torch.float32, device="cuda")
+    outs = []
+    for overlap in (False, True, True):
+        ro = Rollout(ctx, w.grid, w.prox, W.plain_steer((1.0, 0.8, 0.6)), w.n, w.dt, 40, 7, overlap=overlap)
+        ro.forward([1.0, 0.8, 0.6], pos0, fw0, sa0)
+        g = ro.backward(dev(gp), z, z)
+        torch.cuda.synchronize()
+        outs.append([x.cpu().numpy() for x in g])
+    for run in outs[1:]:
+        for a, b in zip(outs[0], run):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.abs(outs[0][1]).max() > 0
