@@ -912,7 +912,12 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
         ~MarkGuard() { brt::launch_mark() = brt::LaunchMark{}; }
     } mark_guard;
     brt::launch_mark() = brt::LaunchMark{};
-    if (ctx->timing && !given) {
+    // no phase timing of a step being captured into a caller's graph (the
+    // kernel-bound events are not graph nodes)
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    if (ctx->timing && !given && hipStreamIsCapturing(s, &capture) != hipSuccess)
+        capture = hipStreamCaptureStatusActive;
+    if (ctx->timing && !given && capture == hipStreamCaptureStatusNone) {
         if (ctx->events_used == 4096) fold_events(ctx);
         if (ctx->events_used == ctx->events.size()) {
             std::array<hipEvent_t, 2 * BRT_NUM_PHASES> a;

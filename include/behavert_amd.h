@@ -522,8 +522,10 @@ int brt_snapshot_load(brt_context* ctx, const char* path, uint32_t flags,
  * Phases: 0 hash+sort, 1 reorder+ranges, 2 neighbours, 3 steering+integration,
  * 4 feature update.  With BRT_OPT_FUSED_STEP, phase 2 holds the fused
  * neighbours+steering kernel and phase 3 is empty; a phase that launched no
- * kernel reads 0.  brt_phase_times() synchronises the recorded events and
- * returns accumulated milliseconds per phase and the number of timed steps. */
+ * kernel reads 0.  Steps enqueued while their stream is being captured into a
+ * graph are not timed (and not counted).  brt_phase_times() synchronises the
+ * recorded events and returns accumulated milliseconds per phase and the
+ * number of timed steps. */
 #define BRT_NUM_PHASES 5
 int brt_phase_timing_enable(brt_context* ctx, int enable);
 int brt_phase_times(brt_context* ctx, double* ms_out, uint32_t* steps_out);
