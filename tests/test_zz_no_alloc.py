@@ -1,8 +1,9 @@
 """brt_step_noalloc (the step with its scratch in a static device array, no
 allocation) against brt_step through the C ABI: two calls in a row on one
-context, the second with a smaller input and another step shape, each equal
-bit for bit to brt_step on the same input; a call above the size limit is
-rejected with an error."""
+context, the second with a smaller input and another step shape (then again
+without the sorted smoothed-acceleration copy), each equal bit for bit to
+brt_step on the same input; a call above the size limit is rejected with an
+error."""
 import ctypes as C
 
 import numpy as np
@@ -28,9 +29,10 @@ def noalloc_step(fl, dt, step, feature_rate=0.0):
         fl.features, fl.features_out = fl.features_out, fl.features
 
 
-def same_step(ctx, w, steps):
+def same_step(ctx, w, steps, sorted_smooth_accel=True):
     a = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander)
-    b = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander)
+    b = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander,
+              sorted_smooth_accel=sorted_smooth_accel)
     for t in range(steps):
         if w.moving_targets:
             a.steer.set_targets(w.targets_at(t))
@@ -61,6 +63,9 @@ def test_noalloc_step_equals_step(ctx):
     w.grid = W.cube_grid(w.side)
     print(f"no-alloc step: {w.name} at {w.n} agents", flush=True)
     same_step(ctx, w, 2)
+    # third call, without the sorted smoothed-acceleration copy
+    print(f"no-alloc step: {w.name} at {w.n} agents, no sorted smoothed acceleration", flush=True)
+    same_step(ctx, w, 2, sorted_smooth_accel=False)
 
 
 def test_noalloc_step_rejects_more_than_its_maximum(ctx):
