@@ -361,6 +361,33 @@ __device__ __forceinline__ void log_round(const ProxArgs& a, Lane& L, uint32_t j
     L.cnt += __popc(m);
 }
 
+// Round log of the flat scan: as log_round, with the self test folded into a
+// mask the caller selects per round (all ones unless the round holds the
+// agent), and the word stored unconditionally, the write pointer advancing only
+// when the word carries a bit.  A word is stored only while the lane holds
+// fewer than K, so at most K words are kept and the store lands at most at
+// row[K] < W; a stored word without bits is overwritten by the next word or
+// by the list's padding.
+template <int G>
+__device__ __forceinline__ void log_round_flat(const ProxArgs& a, Lane& L, uint32_t j, uint32_t rem, uint32_t sm,
+                                               const float4* q) {
+    static_assert(G >= 1 && G <= 8, "a round's bits must fit in 8 bits");
+    uint32_t m = 0;
+#pragma unroll
+    for (int t = 0; t < G; ++t) {
+        const float rx = q[t].x - L.px, ry = q[t].y - L.py, rz = q[t].z - L.pz;
+        const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
+        const float d2 = (xx + yy) + zz;
+        m = shift_in(m, !(d2 > a.T));
+    }
+    const uint32_t r = rem < (uint32_t)G ? rem : (uint32_t)G;
+    m &= ~((1u << ((uint32_t)G - r)) - 1u) & sm;
+    m <<= 8 - G;
+    *L.wp = (j << 8) | m;
+    L.wp += m != 0u ? 1 : 0;
+    L.cnt += __popc(m);
+}
+
 // Expands the round log into the list, last word first and in place: word e
 // sits at row[1+e] and its entries go to row[1+p] for p >= the bits of words
 // 0..e-1 >= e, so no unread word is overwritten.  Entries past K are dropped
@@ -596,12 +623,20 @@ __device__ __forceinline__ void flat_fetch(const ProxArgs& a, const Seg& sg, I3 
     ne = a.cell_end[key];
 }
 
+// own_start: the first sorted index of the agent's own cell.  Rounds of a cell
+// start at its first index and advance by G, so the agent's own index i lies
+// in the round j_self = own_start + G * floor((i - own_start) / G), in slot
+// (i - own_start) mod G, and in no round of another cell.
 template <int G, bool PACKED>
-__device__ __forceinline__ void scan_flat(const ProxArgs& a, const Seg& sg, Lane& L, I3 gp) {
+__device__ __forceinline__ void scan_flat(const ProxArgs& a, const Seg& sg, Lane& L, I3 gp, uint32_t own_start) {
+    static_assert((G & (G - 1)) == 0, "the self round is found with a mask");
     const brt_grid_params& g = sg.g;
     const uint32_t maxBodies = g.max_bodies_per_cell;
     const __amdgpu_buffer_rsrc_t rpos = float4_rsrc(a.spos, a.n);
     const __amdgpu_buffer_rsrc_t rxyz = xyz_rsrc(PACKED ? a.pxyz : nullptr, PACKED ? a.n : 0u);
+    const uint32_t ds = (L.i - own_start) & (uint32_t)(G - 1);
+    const uint32_t jself = L.i - ds;
+    const uint32_t smask = ~(1u << ((uint32_t)G - 1u - ds));
     uint32_t todo = visit_mask(sg, L, gp, a.Tskip);
     // the fetched (next) cell: ordinal nc (27: none left) and its range
     uint32_t nc = 27u, ns = BRT_EMPTY_CELL, ne = 0u;
@@ -632,7 +667,7 @@ __device__ __forceinline__ void scan_flat(const ProxArgs& a, const Seg& sg, Lane
             // candidates past lim are read (zeros past n) and not taken
             float4 q[G];
             load_round<G, PACKED>(rpos, rxyz, j, q);
-            log_round<G, true>(a, L, j, lim - j, L.i - j, q);
+            log_round_flat<G>(a, L, j, lim - j, j == jself ? smask : ~0u, q);
             j += (uint32_t)G;
             if (L.cnt >= a.K) live = false;
         }
@@ -693,11 +728,13 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
         // cell; sparse waves, dominated by per-cell overhead, take the short
         // rounds of kGroupR with one code path
         bool dense = false;
+        uint32_t own_start = 0;
         if (LDSL) {
             uint32_t occ = 0;
             if (alive) {
                 const uint32_t key = a.hash[L.i].x;
-                occ = a.cell_end[key] - a.cell_start[key];
+                own_start = a.cell_start[key];
+                occ = a.cell_end[key] - own_start;
             }
             dense = __popcll(__ballot(alive && occ >= (uint32_t)BRT_PROX_DENSE_OCC)) >= BRT_PROX_DENSE_LANES;
         }
@@ -706,16 +743,16 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
         if (LDSL && BRT_PROX_FLAT && dense && a.log_rounds) {
             if (alive) {
                 if (packed)
-                    scan_flat<kGroupL, true>(a, sg, L, gp);
+                    scan_flat<kGroupL, true>(a, sg, L, gp, own_start);
                 else
-                    scan_flat<kGroupL, false>(a, sg, L, gp);
+                    scan_flat<kGroupL, false>(a, sg, L, gp, own_start);
             }
         } else if (LDSL && BRT_PROX_FLAT_SPARSE && a.log_rounds) {
             if (alive) {
                 if (packed)
-                    scan_flat<kGroupR, true>(a, sg, L, gp);
+                    scan_flat<kGroupR, true>(a, sg, L, gp, own_start);
                 else
-                    scan_flat<kGroupR, false>(a, sg, L, gp);
+                    scan_flat<kGroupR, false>(a, sg, L, gp, own_start);
             }
         } else if (alive) {
             if (dense) {
