@@ -18,14 +18,16 @@
 // T, the largest float with sqrtf(T) <= r; for a correctly rounded, monotone
 // sqrtf, "sqrtf(d2) > r" and "d2 > T" agree for every d2, NaN included.
 //
-// MI355X design.  One lane per sorted agent, 256 lanes per workgroup; lists are
-// staged in LDS and written out with 16-byte coalesced stores.  Per wave64:
+// MI355X design.  One lane per sorted agent, 256 lanes per workgroup (64 for
+// LDS-list launches of 2^18 agents or more, see prox_block); lists are staged
+// in LDS and written out with 16-byte coalesced stores, or gathered in
+// registers as 16-byte chunks.  Per wave64:
 //   * uniform mode (all lanes in few cells): lanes are grouped by cell; a group
 //     walks the shared canonical candidate stream with wave-uniform indices, so
 //     candidate positions come through the scalar cache (s_load);
 //   * flat mode (LDS lists): each lane walks its own stream of cells with its
-//     own cell index, eight (dense waves) or four (sparse waves) candidate
-//     loads in flight per round, accept bits logged per round;
+//     own cell index, eight (dense waves) or four (sparse waves) candidates
+//     per round, their accept bits logged per round;
 //   * vector mode (register-chunk lists): each lane walks its own stream, the
 //     wave in step cell by cell.
 // Every mode skips a cell whose box lies farther than r from the agent (with
