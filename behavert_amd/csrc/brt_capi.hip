@@ -865,6 +865,10 @@ int brt_bc_feature_update(brt_context* ctx, const brt_prox_params* prox, float r
     return BRT_OK;
 }
 
+}  // extern "C"
+
+// step_core and the step graphs: C++ helpers of the step entry points, outside
+// the extern "C" block (they return and take C++ types)
 namespace {
 
 // Scratch of one step given by the caller of step_core instead of the
@@ -1116,6 +1120,8 @@ int capture_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_p
 }
 
 }  // namespace
+
+extern "C" {
 
 int brt_step(brt_context* ctx, const brt_grid_params* grid, const brt_prox_params* prox,
              const brt_steer_params* steer, float feature_rate, float dt, uint32_t step,
