@@ -2,7 +2,8 @@
 
 Every kernel TU is built with -ffp-contract=off (no FMA contraction: the
 reference arithmetic is restated with one rounding per operation) and with the
-packed-FP32 target feature disabled.
+packed-FP32 target feature disabled.  The TUs are compiled one by one (in
+parallel) and linked, so a TU can carry flags of its own (SOURCE_FLAGS).
 """
 from __future__ import annotations
 
@@ -10,6 +11,8 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -23,7 +26,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -31,6 +33,16 @@ HIPCC_FLAGS = [
     "-Wno-unused-function",
     "-Wno-unused-command-line-argument",
 ]
+
+# Per-TU flags.  The neighbour kernels are scheduled for instruction-level
+# parallelism (LLVM's max-ilp machine scheduler): on the 1M-boid blob flock the
+# fused neighbour kernel takes 402 us against 421 us with the default
+# (occupancy-first) scheduler, at the same 108 VGPRs; the steering, clustering
+# and adjoint kernels measured 1-4 % slower with it, so they keep the default.
+# Scheduling reorders instructions only: every result bit is the same.
+SOURCE_FLAGS = {
+    "brt_proximity.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+}
 
 
 def hipcc() -> str:
@@ -62,18 +74,29 @@ def build_example(verbose: bool = False) -> str:
     return EXAMPLE_BIN
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *srcs]
+def _run(cmd, what: str, verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         sys.stderr.write(proc.stdout + proc.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {proc.returncode}")
+        raise RuntimeError(f"{what} failed with exit code {proc.returncode}")
     if verbose and proc.stderr.strip():
         sys.stderr.write(proc.stderr)
+
+
+def build(verbose: bool = False) -> str:
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = LIB + ".tmp"
+    with tempfile.TemporaryDirectory(prefix="brt_build_") as objdir:
+        objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in srcs]
+        cmds = [[hipcc(), *HIPCC_FLAGS, *SOURCE_FLAGS.get(s, []), "-c", "-o", o, os.path.join(CSRC, s)]
+                for s, o in zip(srcs, objs)]
+        jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            for f in [pool.submit(_run, c, f"compiling {s}", verbose) for c, s in zip(cmds, srcs)]:
+                f.result()
+        _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], "linking", verbose)
     os.replace(tmp, LIB)
     build_example(verbose)
     return LIB
