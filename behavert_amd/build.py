@@ -34,14 +34,24 @@ HIPCC_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-# Per-TU flags.  The neighbour kernels are scheduled for instruction-level
-# parallelism (LLVM's max-ilp machine scheduler): on the 1M-boid blob flock the
-# fused neighbour kernel takes 402 us against 421 us with the default
-# (occupancy-first) scheduler, at the same 108 VGPRs; the steering, clustering
-# and adjoint kernels measured 1-4 % slower with it, so they keep the default.
-# Scheduling reorders instructions only: every result bit is the same.
+# Per-TU flags, each measured against the default scheduler on its own
+# workload (A/B on one box): the neighbour kernels (brt_proximity.hip), the
+# sort (brt_envgrid.hip) and the adjoint (brt_adjoint.hip) are scheduled for
+# instruction-level parallelism (LLVM's max-ilp machine scheduler: fused
+# neighbour kernel 421 -> 402 us on the 1M-boid blob flock at the same 108
+# VGPRs, onesweep pass 19.7 -> 19.4 us, adjoint step 56 -> 54 us); the steering
+# and clustering kernels (brt_opensteer.hip, brt_clustering.hip) schedule with
+# the AMDGPU register-pressure trackers (4M full-stack steering + feature pass
+# 832 -> 789 us, similarities 343 -> 324 us), where max-ilp measured 1-4 %
+# slower.  Scheduling reorders instructions only: every result bit is the same.
+_ILP = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
+_TRACKERS = ["-mllvm", "-amdgpu-use-amdgpu-trackers"]
 SOURCE_FLAGS = {
-    "brt_proximity.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+    "brt_proximity.hip": _ILP,
+    "brt_envgrid.hip": _ILP,
+    "brt_adjoint.hip": _ILP,
+    "brt_opensteer.hip": _TRACKERS,
+    "brt_clustering.hip": _TRACKERS,
 }
 
 
