@@ -105,3 +105,16 @@ def test_workload_generators_are_deterministic():
     assert p[:, :3].min() >= 0 and p[:, :3].max() < 100 and np.all(p[:, 3] == 1)
     blobs = W.blob_flock(1 << 14, 400.0, 64, 10.0, 0)
     assert blobs["pos"][:, :3].max() < 400 and blobs["pos"][:, :3].min() >= 0
+
+
+def test_build_flags_name_existing_sources():
+    """Every per-file compiler flag set in the build names a source that is
+    built, and no flag set changes arithmetic (only scheduler options)."""
+    from behavert_amd import build as hb
+    for src, flags in hb.SOURCE_FLAGS.items():
+        assert src in hb.SOURCES and os.path.exists(os.path.join(hb.CSRC, src)), src
+        opts = [f for f in flags if f != "-mllvm"]
+        assert opts and all(o.startswith("-amdgpu-sched-strategy=") or o == "-amdgpu-use-amdgpu-trackers"
+                            for o in opts), (src, flags)
+    for f in ("-ffp-contract=off", "-fno-fast-math", "-packed-fp32-ops"):
+        assert f in hb.HIPCC_FLAGS
