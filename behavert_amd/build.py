@@ -36,10 +36,12 @@ HIPCC_FLAGS = [
 
 # Per-TU flags, each measured against the default scheduler on its own
 # workload (A/B on one box): the neighbour kernels (brt_proximity.hip), the
-# sort (brt_envgrid.hip) and the adjoint (brt_adjoint.hip) are scheduled for
+# sort (brt_envgrid.hip), the adjoint (brt_adjoint.hip) and the list
+# consumers (brt_consumers.hip) are scheduled for
 # instruction-level parallelism (LLVM's max-ilp machine scheduler: fused
 # neighbour kernel 421 -> 402 us on the 1M-boid blob flock at the same 108
-# VGPRs, onesweep pass 19.7 -> 19.4 us, adjoint step 56 -> 54 us); the steering
+# VGPRs, onesweep pass 19.7 -> 19.4 us, adjoint step 56 -> 54 us,
+# steer-and-slow-to-avoid 179 -> 167 us); the steering
 # and clustering kernels (brt_opensteer.hip, brt_clustering.hip) schedule with
 # the AMDGPU register-pressure trackers (4M full-stack steering + feature pass
 # 832 -> 789 us, similarities 343 -> 324 us), where max-ilp measured 1-4 %
@@ -50,6 +52,7 @@ SOURCE_FLAGS = {
     "brt_proximity.hip": _ILP,
     "brt_envgrid.hip": _ILP,
     "brt_adjoint.hip": _ILP,
+    "brt_consumers.hip": _ILP,
     "brt_opensteer.hip": _TRACKERS,
     "brt_clustering.hip": _TRACKERS,
 }
