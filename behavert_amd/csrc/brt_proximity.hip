@@ -265,13 +265,15 @@ __device__ __forceinline__ void append(Lane& L, uint32_t c, bool take) {
 }
 
 // Count and padding (LDS lists), or the chunks not yet stored (register-chunk
-// lists: the pending partial chunk, the all-padding chunks, then chunk 0).
+// lists: the pending partial chunk, then chunk 0).  Register-chunk lists
+// return the row's first all-padding chunk: those chunks are stored by the
+// whole wave together (pad_rows).
 template <bool LDSL>
-__device__ __forceinline__ void finish_list(Lane& L, uint32_t W) {
+__device__ __forceinline__ uint32_t finish_list(Lane& L, uint32_t W) {
     if (LDSL) {
         L.row[0] = L.cnt;
         for (uint32_t q = L.cnt + 1; q < W; ++q) L.row[q] = BRT_LIST_PAD;
-        return;
+        return W / 4u;
     }
     const uint32_t P = BRT_LIST_PAD, k = L.cnt;
     uint4* row = reinterpret_cast<uint4*>(L.row);
@@ -281,8 +283,26 @@ __device__ __forceinline__ void finish_list(Lane& L, uint32_t W) {
         row[q] = make_uint4(rem > 0u ? L.b0 : P, rem > 1u ? L.b1 : P, rem > 2u ? L.b2 : P, P);
         ++q;
     }
-    for (; 4u * q < W; ++q) row[q] = make_uint4(P, P, P, P);
     row[0] = make_uint4(k, k > 0u ? L.h0 : P, k > 1u ? L.h1 : P, k > 2u ? L.h2 : P);
+    return q;
+}
+
+// Register-chunk lists: the all-padding chunks of the wave's rows (consecutive
+// rows of W words from sorted index row0; lane l's row starts padding at its
+// chunk pad_from), stored by the whole wave with consecutive 16-byte chunks per
+// store instead of each lane's own strided row.  The trip count is
+// wave-uniform, so every lane takes part in the shuffle.
+__device__ __forceinline__ void pad_rows(uint32_t* lists, uint32_t row0, uint32_t rows, uint32_t W, uint32_t lane,
+                                         uint32_t pad_from) {
+    const uint32_t W4 = W / 4u, total = rows * W4;
+    const uint32_t P = BRT_LIST_PAD;
+    uint4* dst = reinterpret_cast<uint4*>(lists + (size_t)row0 * W);
+    for (uint32_t base = 0; base < total; base += 64u) {
+        const uint32_t w = base + lane;
+        const uint32_t r = w / W4;
+        const uint32_t first = (uint32_t)__shfl((int)pad_from, (int)(r < 64u ? r : 63u));
+        if (w < total && w - r * W4 >= first) dst[w] = make_uint4(P, P, P, P);
+    }
 }
 
 // Tests the candidates j .. j+G-1 (positions q) for lane L and appends
@@ -786,7 +806,10 @@ __global__ void __launch_bounds__(256) k_prox(ProxArgs a) {
     }
 
     if (LDSL && a.log_rounds) expand_log(L, a.K);
-    if (LDSL || alive) finish_list<LDSL>(L, a.W);
+    uint32_t pad_from = a.W / 4u;
+    if (LDSL || alive) pad_from = finish_list<LDSL>(L, a.W);
+    if (!LDSL && wave_live)
+        pad_rows(a.lists, sg.row0, (sg.end - sg.row0) < 64u ? (sg.end - sg.row0) : 64u, a.W, lane, pad_from);
     if (LDSL) __syncthreads();
     // LDS lists: each wave writes its own rows, 16 bytes per lane per store
     if (LDSL && wave_live) {
