@@ -27,6 +27,13 @@ namespace {
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
+// Predecessor statuses loaded together per look-back step (onesweep pass at
+// 1M keys: 19.4 us with 1, 18.3 with 4, 18.4 with 8, 19.3 with 16; at 4M keys
+// 46.3, 43.5, 44.6 and 48.7 us).
+#ifndef BRT_SORT_LOOKBACK
+#define BRT_SORT_LOOKBACK 4
+#endif
+constexpr int kLookback = BRT_SORT_LOOKBACK;
 
 __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_tmp) {
     const int lane = threadIdx.x & 63;
@@ -237,31 +244,53 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
     const uint32_t gex = block_excl_scan_256(s_gcount[tid], s_tmp);
     s_digit_start[tid] = digit_start;
 
-    // decoupled look-back over the preceding tiles for digit tid
+    // decoupled look-back over the preceding tiles for digit tid, kLookback
+    // tiles per step: the statuses of tiles p, p-1, ..., p-kLookback+1 are
+    // loaded together and taken nearest first (aggregates summed up to the
+    // first inclusive prefix), so a walk over m tiles waits for about
+    // m / kLookback memory latencies instead of m
     uint32_t excl = 0u;
     if (tile > 0) {
         int32_t p = (int32_t)tile - 1;
         uint32_t spins = 0;
         while (p >= 0) {
-            uint32_t v = __hip_atomic_load(status + (size_t)p * kRadix + tid, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t flag = v & ~kValueMask;
-            if (flag == 0u) {
-                // A predecessor tile has started (tile IDs are taken in start order)
-                // and publishes without waiting, so this wait ends; the bound only
-                // turns an impossible stall into a flagged wrong result instead of
-                // a hang: the sticky device flag is raised for the host to report.
+            uint32_t v[kLookback];
+#pragma unroll
+            for (int k = 0; k < kLookback; ++k)
+                v[k] = (p - k >= 0) ? __hip_atomic_load(status + (size_t)(p - k) * kRadix + tid, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT)
+                                    : kFlagInc;  // before tile 0: never reached (tile 0 is inclusive)
+            uint32_t add = 0u, used = 0u;
+            bool inc = false, stall = false;
+#pragma unroll
+            for (int k = 0; k < kLookback; ++k) {
+                const uint32_t flag = v[k] & ~kValueMask;
+                if (!inc && !stall) {
+                    if (flag == 0u) {
+                        stall = true;
+                    } else {
+                        add += v[k] & kValueMask;
+                        used = (uint32_t)k + 1u;
+                        inc = flag == kFlagInc;
+                    }
+                }
+            }
+            excl += add;
+            if (inc) break;
+            p -= (int32_t)used;
+            if (stall) {
+                // A predecessor tile has started (tile IDs are taken in start
+                // order) and publishes without waiting, so this wait ends; the
+                // bound only turns an impossible stall into a flagged wrong
+                // result instead of a hang: the sticky device flag is raised
+                // for the host to report.
                 if (++spins > (1u << 24)) {
                     __hip_atomic_fetch_or(dev_flags, BRT_DEVFLAG_SORT_LOOKBACK, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_SYSTEM);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
-                continue;
             }
-            excl += v & kValueMask;
-            if (flag == kFlagInc) break;
-            --p;
         }
         __hip_atomic_store(my_status, kFlagInc | (excl + tcount), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
