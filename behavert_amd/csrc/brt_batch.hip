@@ -41,6 +41,9 @@ struct brt_batch {
     float* xyz = nullptr;      // sorted positions packed as x,y,z (float[3 total_n])
     uint32_t* meta = nullptr;  // sort histograms / counters / look-back status
     size_t meta_words = 0;
+    // every word of meta is zero once the work enqueued so far has run (zeroed
+    // at creation; each step's sort passes and reorder return what they used)
+    bool meta_clean = false;
     brt_batch_config cfg{};
     hipStream_t cap_stream = nullptr;
     // captured steps, most recently used last; a step with the feature update
@@ -95,14 +98,19 @@ int enqueue_step(brt_batch* b, const brt_steer_params* steer, float feature_rate
     brt_context* ctx = b->ctx;
     const uint32_t n = b->total_n;
     brt::SortPlan plan = brt::make_sort_plan(n, b->total_cells, b->cfg.sort_items);
-    BRT_HIP(brt::launch_prepare(b->meta, brt::sort_meta_words(plan), buf->cell_start, buf->cell_end, b->total_cells,
-                                s),
-            "k_prepare");
+    // the clearing launch only when the metadata is not known to be zero; the
+    // hash kernel fills the empty cells otherwise
+    const bool clean = b->meta_clean;
+    b->meta_clean = false;
+    if (!clean)
+        BRT_HIP(brt::launch_prepare(b->meta, b->meta_words, buf->cell_start, buf->cell_end, b->total_cells, s),
+                "k_prepare");
     brt::BatchView view{b->d_flocks, b->d_wave_flock, b->waves, b->d_step};
     uint2* A = reinterpret_cast<uint2*>(buf->hash);
     uint2* B = reinterpret_cast<uint2*>(b->tmp);
     uint2* first = (plan.passes & 1u) ? B : A;
-    BRT_HIP(brt::launch_batch_hash_hist(view, reinterpret_cast<const float4*>(buf->pos), first, b->meta, plan, s),
+    BRT_HIP(brt::launch_batch_hash_hist(view, reinterpret_cast<const float4*>(buf->pos), first, b->meta, plan, s,
+                                        clean ? buf->cell_start : nullptr, buf->cell_end, b->total_cells),
             "k_batch_hash_hist");
     uint2* src = first;
     uint2* dst = first == A ? B : A;
@@ -125,9 +133,11 @@ int enqueue_step(brt_batch* b, const brt_steer_params* steer, float feature_rate
     // the LDS-list scan reads candidates from the packed x,y,z copy
     float* xyz = b->cfg.lds_lists ? b->xyz : nullptr;
     f.xyz0 = xyz;
+    const brt::SortCleanup cleanup = brt::sort_cleanup(b->meta, plan);
     BRT_HIP(brt::launch_reorder_ranges(reinterpret_cast<const uint2*>(buf->hash), n, f, b->total_cells,
-                                       buf->cell_start, buf->cell_end, s),
+                                       buf->cell_start, buf->cell_end, s, &cleanup),
             "k_reorder_ranges");
+    b->meta_clean = true;
     BRT_HIP(brt::launch_batch_neighbors_steer(
                 view, n, b->prox, brt_radius_threshold(b->prox.search_radius), *steer, dt,
                 reinterpret_cast<const float4*>(buf->sorted_pos), reinterpret_cast<const float4*>(buf->sorted_forward),
@@ -212,7 +222,11 @@ int batch_step(brt_batch* b, const brt_steer_params* steer, float feature_rate, 
     } else if (hit + 1 != b->graphs.size()) {
         std::swap(b->graphs[hit], b->graphs.back());
     }
+    // a graph captured from clean metadata holds no clearing launch: clear
+    // first if the metadata is not known to be zero (each replay leaves it so)
+    if (!b->meta_clean) BRT_HIP(brt::launch_prepare(b->meta, b->meta_words, nullptr, nullptr, 0, s), "k_prepare");
     BRT_HIP(hipGraphLaunch(b->graphs.back().exec, s), "hipGraphLaunch");
+    b->meta_clean = true;
     return BRT_OK;
 }
 
@@ -343,8 +357,10 @@ int brt_batch_create(brt_context* ctx, const brt_flock_desc* flocks, uint32_t nu
                                                                                          b->total_cells, items)));
     if ((e = hipMalloc(&b->tmp, (size_t)(b->total_n > 0 ? b->total_n : 1) * 8u)) != hipSuccess ||
         (e = hipMalloc((void**)&b->xyz, (size_t)(b->total_n > 0 ? b->total_n : 1) * 12u)) != hipSuccess ||
-        (e = hipMalloc((void**)&b->meta, b->meta_words * 4u)) != hipSuccess)
+        (e = hipMalloc((void**)&b->meta, b->meta_words * 4u)) != hipSuccess ||
+        (e = hipMemset(b->meta, 0, b->meta_words * 4u)) != hipSuccess)
         return fail(hip_err(ctx, e, "hipMalloc(batch sort scratch)"));
+    b->meta_clean = true;
     if ((e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking)) != hipSuccess)
         return fail(hip_err(ctx, e, "hipStreamCreateWithFlags"));
     b->cfg.prox_block = default_block(prox->max_neighbors);

@@ -119,6 +119,7 @@ int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan) {
         ctx->meta = nullptr;
         ctx->meta_bytes = 0;
         size_t bytes = need_meta + need_meta / 4 + 4096;
+        ctx->meta_clean = false;
         BRT_HIP(hipMalloc((void**)&ctx->meta, bytes), "hipMalloc(sort meta)");
         ctx->meta_bytes = bytes;
     }
@@ -127,26 +128,37 @@ int ensure_scratch(brt_context* ctx, uint32_t n, const brt::SortPlan& plan) {
 
 // hash (+ histograms) then the onesweep passes; the sorted pairs end in `hash`.
 // meta: sort metadata (sort_meta_words), tmp: ping-pong uint2[n]; both given
-// (meta == nullptr: the context's, allocated as needed).
+// (meta == nullptr: the context's, allocated as needed).  meta_clean: every
+// word of meta is already zero (then no k_prepare runs and the hash kernel
+// fills the empty cells).  The last pass leaves its metadata nonzero: a
+// reorder with brt::sort_cleanup returns it to zero.
 // ev_start / ev_stop (phase timing): bound to the first and the last kernel.
 int run_hash_sort(brt_context* ctx, const brt_grid_params* g, const float* pos, uint32_t n, uint32_t* hash,
                   uint32_t* cell_start, uint32_t* cell_end, hipStream_t s, uint32_t* meta = nullptr,
-                  void* tmp = nullptr, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr) {
+                  void* tmp = nullptr, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr,
+                  bool meta_clean = false) {
     brt::SortPlan plan = brt::make_sort_plan(n, g->num_cells);
+    size_t clear_words = brt::sort_meta_words(plan);
     if (!meta) {
         int rc = ensure_scratch(ctx, n, plan);
         if (rc) return rc;
         meta = ctx->meta;
         tmp = ctx->tmp;
+        meta_clean = meta_clean && ctx->meta_clean;
+        ctx->meta_clean = false;
+        // the whole buffer: an earlier sort of another size may have left words
+        // beyond this plan's
+        clear_words = ctx->meta_bytes / 4u;
     }
     brt::launch_mark().start = ev_start;
-    BRT_HIP(brt::launch_prepare(meta, brt::sort_meta_words(plan), cell_start, cell_end, g->num_cells, s),
-            "k_prepare");
+    if (!meta_clean)
+        BRT_HIP(brt::launch_prepare(meta, clear_words, cell_start, cell_end, g->num_cells, s), "k_prepare");
     uint2* A = reinterpret_cast<uint2*>(hash);
     uint2* B = reinterpret_cast<uint2*>(tmp);
     uint2* first = (plan.passes & 1u) ? B : A;
     if (plan.passes == 0u) brt::launch_mark().stop = ev_stop;
-    BRT_HIP(brt::launch_hash_hist(*g, reinterpret_cast<const float4*>(pos), n, first, meta, plan, s),
+    BRT_HIP(brt::launch_hash_hist(*g, reinterpret_cast<const float4*>(pos), n, first, meta, plan, s,
+                                  meta_clean ? cell_start : nullptr, cell_end),
             "k_hash_hist");
     uint2* src = first;
     uint2* dst = first == A ? B : A;
@@ -385,6 +397,7 @@ int brt_envgrid_calc_hash(brt_context* ctx, const brt_grid_params* grid, const f
     brt::SortPlan plan = brt::make_sort_plan(n, grid->num_cells);
     if ((rc = ensure_scratch(ctx, n, plan))) return rc;
     // the histogram side product lands in scratch and is ignored
+    ctx->meta_clean = false;
     BRT_HIP(brt::launch_prepare(ctx->meta, brt::kMetaStatus, nullptr, nullptr, 0, s), "k_prepare");
     BRT_HIP(brt::launch_hash_hist(*grid, reinterpret_cast<const float4*>(pos), n, reinterpret_cast<uint2*>(hash),
                                   ctx->meta, plan, s),
@@ -404,6 +417,7 @@ int brt_envgrid_sort(brt_context* ctx, uint32_t* hash, uint32_t n, uint32_t num_
     brt::SortPlan plan = brt::make_sort_plan(n, num_cells);
     if (plan.passes == 0) return BRT_OK;
     if ((rc = ensure_scratch(ctx, n, plan))) return rc;
+    ctx->meta_clean = false;
     BRT_HIP(brt::launch_prepare(ctx->meta, brt::sort_meta_words(plan), nullptr, nullptr, 0, s), "k_prepare");
     uint2* A = reinterpret_cast<uint2*>(hash);
     uint2* B = reinterpret_cast<uint2*>(ctx->tmp);
@@ -958,9 +972,10 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
         if ((rc = run_hash_sort(ctx, grid, b->pos, n, b->hash, b->cell_start, b->cell_end, s, given->meta, tmp)))
             return rc;
     } else {
-        // phase 0: prepare (scratch + empty-cell fill) + hash + sort
+        // phase 0: hash (+ empty-cell fill) + sort; the clearing launch only
+        // when the metadata was not left zero by the previous step
         if ((rc = run_hash_sort(ctx, grid, b->pos, n, b->hash, b->cell_start, b->cell_end, s, nullptr, nullptr,
-                                ev ? (*ev)[0] : nullptr, ev ? (*ev)[1] : nullptr)))
+                                ev ? (*ev)[0] : nullptr, ev ? (*ev)[1] : nullptr, true)))
             return rc;
         if (ev) *ev_mask |= 1u;
         tmp = ctx->tmp;
@@ -985,9 +1000,12 @@ int step_core(brt_context* ctx, const brt_grid_params* grid, const brt_prox_para
     f.count = nf;
     f.xyz0 = xyz;
     mark_phase(1);
+    uint32_t* sort_meta = given ? given->meta : ctx->meta;
+    const brt::SortCleanup cleanup = brt::sort_cleanup(sort_meta, brt::make_sort_plan(n, grid->num_cells));
     BRT_HIP(brt::launch_reorder_ranges(reinterpret_cast<const uint2*>(b->hash), n, f, grid->num_cells, b->cell_start,
-                                       b->cell_end, s),
+                                       b->cell_end, s, &cleanup),
             "k_reorder_ranges");
+    if (!given) ctx->meta_clean = true;
     const float T = brt_radius_threshold(prox->search_radius);
     if (!two_pass) {
         // phase 2: neighbour lists + steering + integration in one pass

@@ -52,31 +52,40 @@ __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_
     return wo + x - v;
 }
 
+// Every cell empty (start = end = BRT_EMPTY_CELL) before the reorder writes
+// the occupied ones; four cells per thread with 16-byte stores where aligned.
+__device__ __forceinline__ void fill_empty_cells(uint32_t* __restrict__ cell_start, uint32_t* __restrict__ cell_end,
+                                                 uint32_t num_cells) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t n4 = num_cells / 4u;
+    const uint4 e = make_uint4(BRT_EMPTY_CELL, BRT_EMPTY_CELL, BRT_EMPTY_CELL, BRT_EMPTY_CELL);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        reinterpret_cast<uint4*>(cell_start)[i] = e;
+        reinterpret_cast<uint4*>(cell_end)[i] = e;
+    }
+    for (uint32_t i = n4 * 4u + blockIdx.x * blockDim.x + threadIdx.x; i < num_cells; i += stride) {
+        cell_start[i] = BRT_EMPTY_CELL;
+        cell_end[i] = BRT_EMPTY_CELL;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_prepare(uint32_t* __restrict__ meta, uint32_t meta_words,
                                                  uint32_t* __restrict__ cell_start,
                                                  uint32_t* __restrict__ cell_end, uint32_t num_cells) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < meta_words; i += stride) meta[i] = 0u;
-    if (cell_start) {
-        // four cells per thread with 16-byte stores where aligned
-        const uint32_t n4 = num_cells / 4u;
-        uint4 e = make_uint4(BRT_EMPTY_CELL, BRT_EMPTY_CELL, BRT_EMPTY_CELL, BRT_EMPTY_CELL);
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-            reinterpret_cast<uint4*>(cell_start)[i] = e;
-            reinterpret_cast<uint4*>(cell_end)[i] = e;
-        }
-        for (uint32_t i = n4 * 4u + blockIdx.x * blockDim.x + threadIdx.x; i < num_cells; i += stride) {
-            cell_start[i] = BRT_EMPTY_CELL;
-            cell_end[i] = BRT_EMPTY_CELL;
-        }
-    }
+    if (cell_start) fill_empty_cells(cell_start, cell_end, num_cells);
 }
 
-// genCalcHashD fused with the per-pass digit histograms.
+// genCalcHashD fused with the per-pass digit histograms (and, when cell_start
+// is given, the empty-cell fill).
 __global__ void __launch_bounds__(256) k_hash_hist(brt_grid_params g, const float4* __restrict__ pos,
                                                    uint32_t n, uint2* __restrict__ out,
-                                                   uint32_t* __restrict__ hist, SortPlan plan) {
+                                                   uint32_t* __restrict__ hist, SortPlan plan,
+                                                   uint32_t* __restrict__ cell_start,
+                                                   uint32_t* __restrict__ cell_end) {
     __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+    if (cell_start) fill_empty_cells(cell_start, cell_end, g.num_cells);
     for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) s_hist[t] = 0u;
     __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -99,8 +108,10 @@ __global__ void __launch_bounds__(256) k_hash_hist(brt_grid_params g, const floa
 // cell, so one stable sort orders every flock and keeps each in its own range.
 __global__ void __launch_bounds__(256) k_batch_hash_hist(BatchView b, const float4* __restrict__ pos,
                                                          uint2* __restrict__ out, uint32_t* __restrict__ hist,
-                                                         SortPlan plan) {
+                                                         SortPlan plan, uint32_t* __restrict__ cell_start,
+                                                         uint32_t* __restrict__ cell_end, uint32_t num_cells) {
     __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+    if (cell_start) fill_empty_cells(cell_start, cell_end, num_cells);
     for (uint32_t t = threadIdx.x; t < plan.passes * kRadix; t += blockDim.x) s_hist[t] = 0u;
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
@@ -158,7 +169,10 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
                                                             const uint32_t* __restrict__ ghist,
                                                             uint32_t* __restrict__ status,
                                                             uint32_t* __restrict__ tile_counter,
-                                                            uint32_t* __restrict__ dev_flags) {
+                                                            uint32_t* __restrict__ dev_flags,
+                                                            uint32_t* __restrict__ prev_hist,
+                                                            uint32_t* __restrict__ prev_counter,
+                                                            uint32_t* __restrict__ prev_status) {
     constexpr int TILE = kSortThreads * ITEMS;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_whist[4][kRadix];
@@ -179,6 +193,23 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t wbase = tile * TILE + wave * (64 * ITEMS);
+    // A tile ID past the grid means the counter did not start at zero (the
+    // metadata a step relies on was not returned to zero): no tile is
+    // processed out of bounds; the flag reports the wrong sort.
+    if (tile >= gridDim.x) {
+        if (tid == 0)
+            __hip_atomic_fetch_or(dev_flags, BRT_DEVFLAG_SORT_LOOKBACK, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return;
+    }
+    // the previous pass's metadata back to zero (its tiles use the same IDs)
+    if (prev_status) {
+        prev_status[(size_t)tile * kRadix + tid] = 0u;
+        if (tile == 0u) {
+#pragma unroll
+            for (uint32_t c = 0; c < kHistCopies; ++c) prev_hist[c * kHistWords + tid] = 0u;
+            if (tid == 0) *prev_counter = 0u;
+        }
+    }
 
     uint2 it[ITEMS];
     uint32_t rank[ITEMS];
@@ -331,10 +362,18 @@ __global__ void __launch_bounds__(kSortThreads) k_onesweep(const uint2* __restri
 }
 
 // reorderDataFloat4 for every feature + cell start/end of the sorted keys.
+// With cleanup, the sort's last pass's metadata goes back to zero as well.
 __global__ void __launch_bounds__(256) k_reorder_ranges(const uint2* __restrict__ hash, uint32_t n, ReorderArgs f,
                                                         uint32_t num_cells, uint32_t* __restrict__ cell_start,
-                                                        uint32_t* __restrict__ cell_end) {
+                                                        uint32_t* __restrict__ cell_end, SortCleanup cleanup) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cleanup.hist) {
+        const uint32_t stride = gridDim.x * blockDim.x;
+        for (uint32_t w = i; w < cleanup.status_words; w += stride) cleanup.status[w] = 0u;
+        for (uint32_t w = i; w < kHistCopies * kRadix; w += stride)
+            cleanup.hist[(w / kRadix) * kHistWords + (w % kRadix)] = 0u;
+        if (i == 0) *cleanup.counter = 0u;
+    }
     if (i >= n) return;
     const uint2 h = hash[i];
     if (h.y < n) {
@@ -393,6 +432,17 @@ size_t sort_meta_words(const SortPlan& p) {
     return (size_t)kMetaStatus + (size_t)p.passes * p.tiles * kRadix;
 }
 
+SortCleanup sort_cleanup(uint32_t* meta, const SortPlan& plan) {
+    SortCleanup c{};
+    if (!meta || plan.passes == 0u) return c;
+    const uint32_t last = plan.passes - 1u;
+    c.hist = meta + kMetaHist + last * kRadix;
+    c.counter = meta + kMetaCounters + last;
+    c.status = meta + kMetaStatus + (size_t)last * plan.tiles * kRadix;
+    c.status_words = plan.tiles * kRadix;
+    return c;
+}
+
 LaunchMark& launch_mark() {
     static thread_local LaunchMark m;
     return m;
@@ -407,16 +457,19 @@ hipError_t launch_prepare(uint32_t* meta, size_t meta_words, uint32_t* cell_star
 }
 
 hipError_t launch_hash_hist(const brt_grid_params& g, const float4* pos, uint32_t n, uint2* out, uint32_t* meta,
-                            const SortPlan& plan, hipStream_t s) {
+                            const SortPlan& plan, hipStream_t s, uint32_t* cell_start, uint32_t* cell_end) {
     uint32_t grid = grid_for(n, 256, hist_blocks(n));
-    BRT_LAUNCH(k_hash_hist, dim3(grid), dim3(256), 0, s, g, pos, n, out, meta + kMetaHist, plan);
+    BRT_LAUNCH(k_hash_hist, dim3(grid), dim3(256), 0, s, g, pos, n, out, meta + kMetaHist, plan, cell_start,
+               cell_start ? cell_end : nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_batch_hash_hist(const BatchView& b, const float4* pos, uint2* out, uint32_t* meta,
-                                  const SortPlan& plan, hipStream_t s) {
+                                  const SortPlan& plan, hipStream_t s, uint32_t* cell_start, uint32_t* cell_end,
+                                  uint32_t num_cells) {
     uint32_t grid = grid_for(b.waves, 4, hist_blocks(b.waves * 64u));
-    hipLaunchKernelGGL(k_batch_hash_hist, dim3(grid), dim3(256), 0, s, b, pos, out, meta + kMetaHist, plan);
+    hipLaunchKernelGGL(k_batch_hash_hist, dim3(grid), dim3(256), 0, s, b, pos, out, meta + kMetaHist, plan,
+                       cell_start, cell_start ? cell_end : nullptr, num_cells);
     return hipGetLastError();
 }
 
@@ -435,19 +488,29 @@ hipError_t launch_hist(const uint2* in, uint2* copy_out, uint32_t n, uint32_t* m
 hipError_t launch_onesweep(const uint2* in, uint2* out, uint32_t n, uint32_t pass, const SortPlan& plan,
                            uint32_t* meta, uint32_t* dev_flags, hipStream_t s) {
     uint32_t* status = meta + kMetaStatus + (size_t)pass * plan.tiles * kRadix;
+    uint32_t* prev_hist = pass ? meta + kMetaHist + (pass - 1u) * kRadix : nullptr;
+    uint32_t* prev_counter = pass ? meta + kMetaCounters + (pass - 1u) : nullptr;
+    uint32_t* prev_status = pass ? status - (size_t)plan.tiles * kRadix : nullptr;
     if (plan.items == 8u)
         BRT_LAUNCH(k_onesweep<8>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n, plan.shift[pass],
-                   plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass, dev_flags);
+                   plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass, dev_flags,
+                   prev_hist, prev_counter, prev_status);
     else
         BRT_LAUNCH(k_onesweep<16>, dim3(plan.tiles), dim3(kSortThreads), 0, s, in, out, n, plan.shift[pass],
-                   plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass, dev_flags);
+                   plan.nbits[pass], meta + kMetaHist + pass * kRadix, status, meta + kMetaCounters + pass, dev_flags,
+                   prev_hist, prev_counter, prev_status);
     return hipGetLastError();
 }
 
 hipError_t launch_reorder_ranges(const uint2* hash, uint32_t n, const ReorderArgs& f, uint32_t num_cells,
-                                 uint32_t* cell_start, uint32_t* cell_end, hipStream_t s) {
-    uint32_t grid = (n + 255) / 256;
-    BRT_LAUNCH(k_reorder_ranges, dim3(grid), dim3(256), 0, s, hash, n, f, num_cells, cell_start, cell_end);
+                                 uint32_t* cell_start, uint32_t* cell_end, hipStream_t s, const SortCleanup* cleanup) {
+    const SortCleanup c = cleanup ? *cleanup : SortCleanup{};
+    uint32_t work = n;
+    if (c.hist && work < c.status_words) work = c.status_words;
+    if (c.hist && work < kHistCopies * kRadix) work = kHistCopies * kRadix;
+    uint32_t grid = (work + 255) / 256;
+    if (grid == 0) return hipSuccess;
+    BRT_LAUNCH(k_reorder_ranges, dim3(grid), dim3(256), 0, s, hash, n, f, num_cells, cell_start, cell_end, c);
     return hipGetLastError();
 }
 

@@ -18,6 +18,10 @@ struct brt_context {
     size_t tmp_bytes = 0;
     uint32_t* meta = nullptr;   // histograms, tile counters, look-back status
     size_t meta_bytes = 0;
+    // every word of meta is zero once the work enqueued so far has run: the
+    // last sort on it was a step's, whose passes and reorder return what they
+    // used to zero; then the next step's sort needs no clearing launch
+    bool meta_clean = false;
     float* partials = nullptr;  // adjoint: per-workgroup weight-gradient partials
     size_t partials_bytes = 0;
     float* xyz = nullptr;       // brt_step: the sorted positions packed as x,y,z

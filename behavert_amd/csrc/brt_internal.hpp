@@ -64,11 +64,16 @@ size_t sort_meta_words(const SortPlan& p);
 // envgrid
 hipError_t launch_prepare(uint32_t* meta, size_t meta_words, uint32_t* cell_start, uint32_t* cell_end,
                           uint32_t num_cells, hipStream_t s);
+// cell_start / cell_end (optional): filled with BRT_EMPTY_CELL in the same
+// launch (when no k_prepare ran before it)
 hipError_t launch_hash_hist(const brt_grid_params& g, const float4* pos, uint32_t n, uint2* out,
-                            uint32_t* meta, const SortPlan& plan, hipStream_t s);
+                            uint32_t* meta, const SortPlan& plan, hipStream_t s, uint32_t* cell_start = nullptr,
+                            uint32_t* cell_end = nullptr);
 hipError_t launch_hist(const uint2* in, uint2* copy_out, uint32_t n, uint32_t* meta, const SortPlan& plan,
                        hipStream_t s);
-// dev_flags: the context's sticky device flag word (BRT_DEVFLAG_*)
+// dev_flags: the context's sticky device flag word (BRT_DEVFLAG_*).  Pass p >= 1
+// also returns the previous pass's histogram slice, tile counter and status
+// words to zero (nothing reads them any more).
 hipError_t launch_onesweep(const uint2* in, uint2* out, uint32_t n, uint32_t pass, const SortPlan& plan,
                            uint32_t* meta, uint32_t* dev_flags, hipStream_t s);
 
@@ -79,8 +84,20 @@ struct ReorderArgs {
     uint32_t count;
     float* xyz0;  // optional: feature 0's x,y,z packed, 12 bytes per agent
 };
+// The sort metadata a finished sort leaves nonzero: the last pass's histogram
+// slice (every copy), tile counter and status words.  A reorder launched right
+// after the sort returns them to zero, so that the whole metadata buffer is
+// zero between steps and the next step needs no clearing launch.
+struct SortCleanup {
+    uint32_t* hist;     // meta + kMetaHist + last * kRadix (copies kHistWords apart); NULL: nothing to clear
+    uint32_t* counter;  // meta + kMetaCounters + last
+    uint32_t* status;   // the last pass's status words
+    uint32_t status_words;
+};
+SortCleanup sort_cleanup(uint32_t* meta, const SortPlan& plan);
 hipError_t launch_reorder_ranges(const uint2* hash, uint32_t n, const ReorderArgs& f, uint32_t num_cells,
-                                 uint32_t* cell_start, uint32_t* cell_end, hipStream_t s);
+                                 uint32_t* cell_start, uint32_t* cell_end, hipStream_t s,
+                                 const SortCleanup* cleanup = nullptr);
 
 // proximity
 hipError_t launch_neighbors(const brt_grid_params& g, const brt_prox_params& p, float threshold,
@@ -190,7 +207,8 @@ inline void cell_slack(const brt_grid_params& g, float out[3]) {
 }
 
 hipError_t launch_batch_hash_hist(const BatchView& b, const float4* pos, uint2* out, uint32_t* meta,
-                                  const SortPlan& plan, hipStream_t s);
+                                  const SortPlan& plan, hipStream_t s, uint32_t* cell_start = nullptr,
+                                  uint32_t* cell_end = nullptr, uint32_t num_cells = 0);
 hipError_t launch_set_word(uint32_t* p, uint32_t v, hipStream_t s);
 hipError_t launch_batch_neighbors_steer(const BatchView& b, uint32_t n, const brt_prox_params& p, float threshold,
                                         const brt_steer_params& sp, float dt, const float4* spos, const float4* sfwd,

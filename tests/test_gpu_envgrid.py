@@ -145,3 +145,50 @@ def test_sort_lookback_flag_is_reported(ctx):
     assert L.brt_context_check(ctx.handle) == 0
     fl.step(w.dt, 1)
     fl.state()
+
+
+def test_steps_keep_sort_metadata_zero_across_standalone_sorts():
+    """A step leaves the sort metadata zero (its passes and reorder return what
+    they used), so the next step skips the clearing launch; a standalone sort
+    leaves it dirty, and the next step clears the whole buffer.  Steps of
+    two sizes interleaved with a larger standalone sort and a standalone hash
+    on one context give the same bits as the same steps each on a fresh context."""
+    from behavert_amd.engine import Context, Flock
+
+    big, small = W.blobs_1m(), W.plain_4096()
+    shared = Context(0)
+    fb = Flock(shared, big.grid, big.prox, big.steer, big.state)
+    fs = Flock(shared, small.grid, small.prox, small.steer, small.state)
+    ref_b = Flock(Context(0), big.grid, big.prox, big.steer, big.state)
+    ref_s = Flock(Context(0), small.grid, small.prox, small.steer, small.state)
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 60_000, 2_000_000).astype(np.uint32)
+    pairs = dev(np.stack([keys, np.arange(len(keys), dtype=np.uint32)], 1).view(np.int32))
+
+    def same(a, b):
+        for k in ("hash", "lists", "cell_start", "cell_end"):
+            x, y = a.grid_outputs()[k], b.grid_outputs()[k]
+            assert np.array_equal(x, y), k
+        sa, sb = a.state(), b.state()
+        for k in sa:
+            assert np.array_equal(np.asarray(sa[k]).view(np.uint32), np.asarray(sb[k]).view(np.uint32)), k
+
+    fb.step(big.dt, 0)
+    ref_b.step(big.dt, 0)
+    same(fb, ref_b)
+    shared.sort(pairs, 60_000)              # 2M keys: more tiles than the 1M step uses
+    fs.step(small.dt, 0)                     # smaller step after a dirty standalone sort
+    ref_s.step(small.dt, 0)
+    same(fs, ref_s)
+    fb.step(big.dt, 1)                       # the larger step after the smaller one
+    ref_b.step(big.dt, 1)
+    same(fb, ref_b)
+    hd = empty_u32(small.n, 2)
+    shared.calc_hash(small.grid, dev(small.state["pos"]), hd)   # histogram side product
+    fb.step(big.dt, 2)
+    ref_b.step(big.dt, 2)
+    same(fb, ref_b)
+    fs.step(small.dt, 1)
+    ref_s.step(small.dt, 1)
+    same(fs, ref_s)
+    shared.check_device()
