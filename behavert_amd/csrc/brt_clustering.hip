@@ -190,7 +190,7 @@ __device__ __forceinline__ void similarity_agent(uint32_t i, uint32_t K, uint32_
     }
     ClusterAccum acc;
     acc.init();
-    const uint32_t count = for_list_octets<!STEER>(lists, W, K, i, n, [&](uint32_t base, const uint32_t(&jj)[8], uint32_t mask) {
+    const uint32_t count = for_list_octets<STEER ? kRowPrefetch : kRowWhole>(lists, W, K, i, n, [&](uint32_t base, const uint32_t(&jj)[8], uint32_t mask) {
 #pragma unroll
         for (int g0 = 0; g0 < 8; g0 += G) {
             float4 other[G][FQ];

@@ -113,10 +113,10 @@ __global__ void __launch_bounds__(256) k_steer(brt_grid_params g, uint32_t W, br
     uint32_t count = 0;
     if (any) {
         // the list in order, eight entries at a time with their gathers in flight;
-        // the feature variants hold four more gathers per entry and keep the
-        // pairwise row reads (whole K 32 rows: 213 -> 162 us for the two-kernel
+        // the feature variants hold four more gathers per entry and read the
+        // row in pairs (whole K 32 rows: 213 -> 162 us for the two-kernel
         // step's steering on 1M blobs, 185 -> 117 us for the behaviour forces)
-        count = for_list_octets<!FEAT>(lists, W, W - 1u, i, n, [&](uint32_t, const uint32_t(&jj)[8], uint32_t mask) {
+        count = for_list_octets<FEAT ? kRowPairs : kRowWhole>(lists, W, W - 1u, i, n, [&](uint32_t, const uint32_t(&jj)[8], uint32_t mask) {
             float4 qp[8], qf[8], fq[8];
             uint32_t jid[8];
 #pragma unroll
