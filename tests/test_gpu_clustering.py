@@ -90,11 +90,12 @@ def test_similarities_bit_exact(ctx, orc, F, adaptive):
     assert o["lists"][:, 0].max() > 0
 
 
-@pytest.mark.parametrize("K", [7, 31, 64, 100])
+@pytest.mark.parametrize("K", [7, 31, 32, 64, 100])
 def test_similarities_list_widths(ctx, orc, K):
     """List widths on both sides of the LDS row-tile switch (256 agents up to
-    K = 39, one wave above) and with an even row length (K + 1 = 32); the
-    one-pass similarities + cluster steering as well."""
+    K = 39, one wave above), with an even row length (K + 1 = 32) and with the
+    36-word rows the list walk loads whole (K = 32); the one-pass similarities
+    + cluster steering as well."""
     w = mixed_workload(n=20000, K=K, seed=K)
     o = oracle_grid_outputs(orc, w.grid, w.prox, w.state)
     F = 5
@@ -117,12 +118,14 @@ def test_similarities_list_widths(ctx, orc, K):
     assert first_mismatch(f32(f1).view(np.uint32), rf.view(np.uint32)) is None
 
 
+@pytest.mark.parametrize("K", [16, 32])
 @pytest.mark.parametrize("variant", range(4))
-def test_decomposed_steering_equals_fused(ctx, orc, variant):
+def test_decomposed_steering_equals_fused(ctx, orc, variant, K):
     """steer_forces -> apply_steering_force reproduces the fused steering pass bit
     for bit (the reference's kernel-per-behaviour sequence summing onto the
-    steer force), and the behaviour force equals the FP32 oracle's."""
-    w = mixed_workload()
+    steer force), and the behaviour force equals the FP32 oracle's; K 32 rows
+    (36 words) are the ones the list walk loads whole."""
+    w = mixed_workload(K=K)
     o = oracle_grid_outputs(orc, w.grid, w.prox, w.state)
     sp = [
         SteerParams.make(weights=(1.0, 0.8, 0.6)),
@@ -297,12 +300,14 @@ def test_clustering_step_bit_exact_and_fp64(ctx, orc, adaptive, record_property)
         assert err[ok].max() < 1e-5, k
 
 
+@pytest.mark.parametrize("K", [16, 32])
 @pytest.mark.parametrize("iterations", [0, 1, 2, 10, 25])
-def test_cluster_membership_bit_exact(ctx, orc, iterations):
+def test_cluster_membership_bit_exact(ctx, orc, iterations, K):
     """computeClusterMembership (reset + Jacobi min-label iterations +
     connector) against the oracle's by-ID restatement, labels and connector bit
-    for bit; .y/.z carried; in-place update gives the same labels."""
-    w = mixed_workload(n=30000, K=16, seed=13)
+    for bit; .y/.z carried; in-place update gives the same labels.  K 16 rows
+    are read in prefetched pairs, K 32 rows whole."""
+    w = mixed_workload(n=30000, K=K, seed=13)
     o = oracle_grid_outputs(orc, w.grid, w.prox, w.state)
     n = w.n
     m_in = np.random.default_rng(2).uniform(-1, 1, (n, 3)).astype(np.float32)
@@ -320,9 +325,10 @@ def test_cluster_membership_bit_exact(ctx, orc, iterations):
     assert first_mismatch(f32(m_io).view(np.uint32), rm.view(np.uint32)) is None
 
 
+@pytest.mark.parametrize("K", [16, 32])
 @pytest.mark.parametrize("radius", [1.5, 4.0])
-def test_cluster_hit_rate_bit_exact(ctx, orc, radius):
-    w = mixed_workload(n=30000, K=16, seed=13)
+def test_cluster_hit_rate_bit_exact(ctx, orc, radius, K):
+    w = mixed_workload(n=30000, K=K, seed=13)
     o = oracle_grid_outputs(orc, w.grid, w.prox, w.state)
     m0 = np.random.default_rng(4).uniform(-1, 1, (w.n, 3)).astype(np.float32)
     m, _ = orc.cluster_membership(w.prox, 10, o["hash"], o["lists"], o["sorted_pos"], m0)
