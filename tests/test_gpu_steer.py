@@ -109,6 +109,21 @@ def test_feature_update_bit_exact_and_vs_fp64(ctx, orc):
     assert floored_rel(got, r64, 1.0).max() < 1e-6
 
 
+def test_feature_update_whole_rows_bit_exact(ctx, orc):
+    """K 32 lists (36-word rows, loaded whole by the list walk), dense enough
+    that many rows are full."""
+    from .test_gpu_clustering import mixed_workload
+
+    w = mixed_workload(n=30000, K=32, seed=3)
+    o = oracle_grid_outputs(orc, w.grid, w.prox, w.state)
+    assert (o["lists"][:, 0] == 32).any()
+    feats = np.random.default_rng(3).random((w.n, 4)).astype(np.float32)
+    fout = empty_f32(w.n, 4)
+    ctx.feature_update(w.prox, 0.25, dev(o["hash"].view(np.int32)), dev(o["lists"].view(np.int32)), dev(feats), fout)
+    ref = orc.feature_update_f32(w.prox, 0.25, o["hash"], o["lists"], feats)
+    assert first_mismatch(f32(fout).view(np.uint32), ref.view(np.uint32)) is None
+
+
 def run_gpu_steps(ctx, w, steps, fused=True):
     fl = Flock(ctx, w.grid, w.prox, w.steer, w.state, features=w.features, wander=w.wander)
     for i in range(steps):
